@@ -1,0 +1,221 @@
+/*
+ * semsks.h -- C ABI of the MI355X (gfx950) SEM time loop.
+ *
+ * Drop-in for the spectral-element solve of s-sks (semsyn/src/sem2d_psvsh.c,
+ * sem2d.c, sem2dMP20140816.c).  The reference has no library API: it is a
+ * process (`sem2d_psvsh par=run.par`, applications/test/prem_epi83.pl:81-82)
+ * whose loop body is a set of K&R functions over globals.  Each entry point
+ * below names the reference code it replaces.
+ *
+ * Conventions
+ *   - Plain C types only; host buffers are owned by the caller, device memory
+ *     by the library.  No entry point calls exit().
+ *   - Fields are structure-of-arrays per component (u_x, u_y, u_z), node
+ *     index iz*nx + ix with iz = 0 at the bottom of the box and iz = nz-1 at
+ *     the free surface (sem2d_psvsh.c:1176,1214).
+ *   - One host thread per engine; all device work of an engine is ordered on
+ *     one HIP stream.
+ */
+#ifndef SEMSKS_H
+#define SEMSKS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference ends the process instead; the mapping is:
+   EIO   <- open_file() exit(-1)          (readmodel.c:15-19)
+   ENP   <- np > 1000 exit(-2)            (sem2d_psvsh.c:725-728)
+   ERANGE<- lip() model x out of range    (sem2d_psvsh.c:835-839)
+   EPAR  <- mstpar missing value exit(100)(getpar.c:398-434,715-725)
+   ESTOP <- STOP in the input exit(101)   (getpar.c:378-390)            */
+typedef enum {
+    SEMSKS_OK = 0,
+    SEMSKS_EINVAL = 1,
+    SEMSKS_EIO = 2,
+    SEMSKS_ENOHBM = 3,      /* device memory budget exceeded */
+    SEMSKS_ENONFINITE = 4,  /* state became non-finite */
+    SEMSKS_ERANGE = 5,
+    SEMSKS_ENP = 6,
+    SEMSKS_EDEVICE = 7,     /* HIP runtime error */
+    SEMSKS_ENOSLOT = 8,     /* no free instance slot */
+    SEMSKS_EPAR = 100,
+    SEMSKS_ESTOP = 101
+} semsks_status;
+
+/* Which reference solver's boundary handling to reproduce. */
+enum {
+    SEMSKS_SOLVER_PSVSH = 0,  /* sem2d_psvsh.c: u,v,w injected, GRT_Interface4,
+                                 P-SV and SH interpolation share one slot ring */
+    SEMSKS_SOLVER_PSV = 1,    /* sem2d.c: u,w injected, GRT_Interface4 */
+    SEMSKS_SOLVER_PSV_MP = 2  /* sem2dMP20140816.c: u,w injected, GRT_Interface3 */
+};
+
+/* problem flags */
+enum {
+    SEMSKS_F_NO_KAHAN = 1     /* plain FP32 displacement update (for error studies) */
+};
+
+/* Number of floats in one material record: rho followed by the 15 distinct
+   elastic constants that enter 2-D stress, Voigt pairs
+   (0,0)(0,2)(0,3)(0,4)(0,5)(2,2)(2,3)(2,4)(2,5)(3,3)(3,4)(3,5)(4,4)(4,5)(5,5)
+   with Voigt 0=xx 1=yy 2=zz 3=yz 4=xz 5=xy (inv_index, sem2d_psvsh.c:1584-1597). */
+#define SEMSKS_MAT_STRIDE 16
+
+/* One SEM instance, fully described in memory (what main() of
+   sem2d_psvsh.c:82-519 holds after Parts 1-6.0).  All pointers are host
+   pointers owned by the caller; the library copies what it needs. */
+typedef struct semsks_problem {
+    int nex, nez;          /* elements in x and z */
+    int hpn;               /* polynomial degree, 3..6 (NGLL = hpn+1) */
+    int neh;               /* elements between edge and GRT interface */
+    float eh;              /* element size in km after discret()'s adjustment */
+    float sdt;             /* time step, s */
+    int nst;               /* time levels recorded (steps run = nst-1) */
+    int solver;            /* SEMSKS_SOLVER_* */
+    int absorb;            /* Lysmer edges on (1) or A = 0 (0) */
+    int flags;             /* SEMSKS_F_* */
+    /* material: node -> record id, records of SEMSKS_MAT_STRIDE floats */
+    int n_mat;
+    const float *mat_table;     /* [n_mat][16] */
+    const uint16_t *mat_id;     /* [nz][nx] */
+    /* receivers (kirrecord_zhl node sampling) */
+    int ntrace;
+    const int *rec_ix, *rec_iz;
+    /* GRT boundary records, greenfile layout (time-major, headerless):
+       psv [nt][2(nx+nz)] = u_bot|w_bot|u_left|w_left (w positive down),
+       sh  [nt][nx+nz]    = v_bot|v_left.  NULL means all zero. */
+    int nt;
+    float dp;
+    const float *psv, *sh;
+} semsks_problem;
+
+/* ---------------------------------------------------------------------- */
+/* Host-side inputs: the reference's parameter and file formats            */
+/* ---------------------------------------------------------------------- */
+typedef struct semsks_model semsks_model;
+
+/* Replaces main()'s parameter block and Parts 1-2/6.0 input reading
+   (sem2d_psvsh.c:154-205, discret 534-828, receivers 330-370, GRT files
+   372-396): getpar semantics (name=value, par=file recursion, last
+   definition wins, mstpar), raymodel (readmodel.c:43-109), semmodel
+   (sem2d_psvsh.c:700-731) with per-node layer assignment, rec_file,
+   greenfile/shgreenfile.  Extension keys: solver=0|1|2, semdip=file
+   (six tilted-axis constants per semmodel layer), outdir=dir. */
+semsks_status semsks_load_par(int argc, char **argv, semsks_model **out);
+const semsks_problem *semsks_model_problem(const semsks_model *m);
+/* derived sizes: nx, nz, mtotal=2(nx+nz), shtotal=nx+nz, nt_sem, plot_trace */
+void semsks_model_dims(const semsks_model *m, int dims[8]);
+const char *semsks_model_outdir(const semsks_model *m);
+void semsks_model_free(semsks_model *m);
+
+/* Writes seis.u / seis.w / seis.v (nst records of ntrace float32 each) from
+   a trace block [nst][3][ntrace] ordered u, -w, v (kirrecord_zhl,
+   sem2d_psvsh.c:2116-2165). */
+semsks_status semsks_write_seis(const char *outdir, int nst, int ntrace, const float *traces);
+
+/* Human-readable text of the last error on this thread. */
+const char *semsks_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* Engine: device state for up to max_slots instances stepped together     */
+/* ---------------------------------------------------------------------- */
+typedef struct semsks_engine semsks_engine;
+
+typedef struct semsks_engine_opts {
+    size_t hbm_budget;     /* bytes the engine may allocate (0 = 90% of free) */
+    int max_slots;         /* instance slots (>= 1) */
+    int tile_z;            /* element rows per wave tile (0 = autotuned/default) */
+    int waves_per_cu;      /* persistent grid size factor (0 = default) */
+    int use_graph;         /* capture the step in a hipGraph (1) or not (0) */
+} semsks_engine_opts;
+
+semsks_status semsks_engine_create(const semsks_engine_opts *opts, semsks_engine **out);
+void semsks_engine_destroy(semsks_engine *e);
+
+/* Admits one instance: uploads the problem, runs the FP64 one-off setup on
+   the device (GLL tables, Jacobian check, lumped mass and its inverse,
+   Lysmer coefficients: sem2d_psvsh.c:218-325), precomputes the interpolated
+   and guarded boundary series (interpl.c:16-91, sem2d_psvsh.c:428-444) and
+   sets time level 0 (GRT_Interface, sem2d_psvsh.c:446-452).  If share_slot
+   >= 0 the instance reuses that slot's mesh and material (same nex, nez,
+   hpn, eh, sdt and material required).  Fails with ENOHBM when the slot's
+   footprint does not fit the budget. */
+semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share_slot, int *slot);
+
+/* Device bytes one problem needs (for admission control). */
+size_t semsks_footprint(const semsks_problem *p, int shared_material);
+
+/* Advances every active instance by up to nsteps time steps (the loop body
+   of sem2d_psvsh.c:418-512: GRT_Interface4/3, Stiff_Matrix,
+   Time_diff_hybrid + Absorbs, the level rotation and kirrecord_zhl).
+   Instances stop by themselves at their nst. Asynchronous unless sync. */
+semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync);
+
+/* Current time level of a slot (-1 if empty) and whether it finished. */
+semsks_status semsks_slot_status(semsks_engine *e, int slot, int *level, int *done);
+
+/* Copies traces [n][3][ntrace] of levels it0..it0+n-1 of a slot to host. */
+semsks_status semsks_get_traces(semsks_engine *e, int slot, int it0, int n, float *out);
+
+/* Copies the current displacement field of a slot to host as three SoA
+   planes [3][nz][nx] (u_x, u_y, u_z). */
+semsks_status semsks_get_field(semsks_engine *e, int slot, float *out);
+
+/* Frees a slot (its device memory returns to the budget). */
+semsks_status semsks_retire(semsks_engine *e, int slot);
+
+/* Blocks until the engine's stream is idle. */
+semsks_status semsks_sync(semsks_engine *e);
+
+/* Milliseconds of device time of the last synchronous semsks_advance call,
+   measured with HIP events on the engine's stream: out[0] = the whole call,
+   out[1] = the element-force/update kernel launches only (summed over the
+   timed steps; eager mode, use_graph = 0, first 1024 steps). */
+semsks_status semsks_last_timing(semsks_engine *e, double out[2]);
+/* Number of steps whose kernel time entered out[1] of semsks_last_timing. */
+int semsks_last_timed_steps(semsks_engine *e);
+
+/* ---------------------------------------------------------------------- */
+/* Single-run convenience (what the CLI uses)                              */
+/* ---------------------------------------------------------------------- */
+/* Runs a whole problem: engine with one slot, nst levels, traces to host
+   [nst][3][ntrace]. */
+semsks_status semsks_run_problem(const semsks_problem *p, float *traces);
+
+/* ---------------------------------------------------------------------- */
+/* Kernel-level entry points (parity tests of single functions)            */
+/* ---------------------------------------------------------------------- */
+/* K u of Stiff_Matrix (sem2d_psvsh.c:1173-1256) for an arbitrary field of
+   slot's mesh and material: u, f host arrays [3][nz][nx].  Runs the same
+   element-force code as the time step. */
+semsks_status semsks_stiff(semsks_engine *e, int slot, const float *u, float *f);
+
+/* Setup products of a slot, to host: lumped mass M (FP64, [nz][nx]), the
+   FP32 time-loop factor dt^2/M ([nz][nx]), and the Lysmer coefficients A
+   (FP64, [nz][nx][3], zero off the edges). */
+semsks_status semsks_get_setup(semsks_engine *e, int slot, double *mass, float *mdt2, double *ab);
+
+/* The interpolated and guarded boundary vectors the time loop injects:
+   g [n][2(nx+nz)], gv [n][nx+nz] for levels 0..n-1 (n <= nst). */
+semsks_status semsks_get_boundary_series(semsks_engine *e, int slot, int n, float *g, float *gv);
+
+/* Fills a problem's boundary records on the device with a closed-form
+   plane S wave (the stand-in for the GRT producer, aserpsv_sem.c:319-488):
+   ray parameter p (s/km), bottom-layer S speed beta (km/s), Ricker f0 (Hz),
+   polarisation psi (deg; SV share cos psi, SH share sin psi).  nt records
+   spaced dp.  Writes the host arrays psv [nt][2(nx+nz)], sh [nt][nx+nz]
+   when they are not NULL; with both NULL the records stay on the device and
+   are used by the next semsks_admit on this engine with p->psv == NULL and
+   p->sh == NULL and p->nt == nt. */
+semsks_status semsks_planewave(semsks_engine *e, int nex, int nez, int hpn, int neh, float eh,
+                               int nt, float dp, double p, double beta, double f0, double psi_deg,
+                               float *psv, float *sh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEMSKS_H */

@@ -1,0 +1,1492 @@
+/*
+ * semsks_device.hip -- gfx950 kernels and the engine behind include/semsks.h.
+ *
+ * One time step of one instance (semsyn/src/sem2d_psvsh.c:418-512):
+ *   inject_kernel  GRT_Interface4 / GRT_Interface3 (sem2d_psvsh.c:2076-2105,
+ *                  sem2dMP20140816.c:2001-2037) on the interface row/column
+ *   step_kernel    Stiff_Matrix (sem2d_psvsh.c:1173-1256: dUj_di_2D, Stress_ij,
+ *                  F_ik, Grid_Stiff) fused with Time_diff_hybrid + Absorbs
+ *                  (sem2d_psvsh.c:1004-1029, 1834-1889) and the level rotation
+ *                  (Copy_Displacement x2, :468-469, replaced by ping-pong)
+ *   record_kernel  kirrecord_zhl (sem2d_psvsh.c:2116-2165)
+ *
+ * The update is written in increment form: with delta_n = u_{n+1} - u_n,
+ *   interior : delta_n = delta_{n-1} - (dt^2/M) f(u_n)
+ *   edges    : delta_n = ((M-A) delta_{n-1} - dt^2 f) / (M+A)
+ * which is algebraically the reference's u+ = -(dt^2/M) f + 2u - u- and its
+ * absorbing rows; u_{n+1} = u_n + delta_n is a Kahan-compensated sum.
+ *
+ * Element force: one 64-lane wave owns a tile of 63 element columns x TZ
+ * element rows.  Lane L computes element column x0-1+L (lane 0 is a halo
+ * column whose only output is its shared right column), sweeping the rows
+ * upward from z0-1 (a halo row whose only output is its shared top row).
+ * Shared nodes are assembled in registers: the top row of an element is
+ * carried to the next row, the right column is handed to the next lane with
+ * a cross-lane shift.  Every node is written by exactly one lane, so the
+ * assembly needs neither atomics nor a colouring pass, and the update is
+ * applied in the same kernel while the element's u is still in registers.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include <algorithm>
+
+#include "semsks.h"
+#include "semsks_internal.h"
+#include "semsks_kernels.h"
+
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            return semsks_set_error(SEMSKS_EDEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #x, \
+                                    hipGetErrorString(e_));                               \
+    } while (0)
+
+/* ========================================================================= */
+/* FP64 one-off setup                                                        */
+/* ========================================================================= */
+
+/* GLL nodes (closed forms, lagrange.c:323-366) and weights
+   2/(n(n+1) P_n(x)^2) (lagrange.c:256-278), in double */
+__host__ __device__ static void gll64(int n, double *x, double *w)
+{
+    if (n == 3) { double r = sqrt(1.0 / 5.0); x[0] = -1; x[1] = -r; x[2] = r; x[3] = 1; }
+    else if (n == 4) { double r = sqrt(15.0 / 35.0); x[0] = -1; x[1] = -r; x[2] = 0; x[3] = r; x[4] = 1; }
+    else {
+        double a = n == 5 ? 315.0 : 1386.0, c = n == 5 ? -210.0 : -1260.0, e = n == 5 ? 15.0 : 210.0;
+        double d = sqrt(c * c - 4 * a * e);
+        double r1 = sqrt((-c + d) / 2 / a), r2 = sqrt((-c - d) / 2 / a);
+        x[0] = -1; x[1] = -r1; x[2] = -r2;
+        if (n == 5) { x[3] = r2; x[4] = r1; x[5] = 1; }
+        else { x[3] = 0; x[4] = r2; x[5] = r1; x[6] = 1; }
+    }
+    for (int i = 0; i <= n; ++i) {
+        if (i == 0 || i == n) { w[i] = 2.0 / (n * (n + 1)); continue; }
+        double p2 = 1, p1 = x[i], pn = 0;
+        for (int k = 2; k <= n; ++k) { pn = ((2 * k - 1) * x[i] * p1 - (k - 1) * p2) / k; p2 = p1; p1 = pn; }
+        w[i] = 2.0 / (n * (n + 1) * pn * pn);
+    }
+}
+
+/* l'_k(x_j) by the product rule (dLag_N_i, lagrange.c:42-67), in double */
+__host__ __device__ static double dlag64(int n, int k, const double *x, double s)
+{
+    double down = 1.0, v = 0.0;
+    for (int i = 0; i <= n; ++i) if (i != k) down *= (x[k] - x[i]);
+    for (int i = 0; i <= n; ++i) {
+        if (i == k) continue;
+        double L = 1.0;
+        for (int m = 0; m <= n; ++m) if (m != k && m != i) L *= (s - x[m]);
+        v += L;
+    }
+    return v / down;
+}
+
+/* element operators of one instance, FP64 -> FP32 (one thread) */
+__global__ void setup_tables_kernel(InstDesc *D, double *w64, double eh)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int n = D->hpn;
+    double x[SEMSKS_MAXNP], w[SEMSKS_MAXNP];
+    gll64(n, x, w);
+    const double jei = 2.0 / eh, half = eh / 2.0;
+    for (int a = 0; a <= n; ++a) {
+        w64[a] = w[a];
+        D->cw[a] = (float)(half * w[a]);
+        for (int t = 0; t <= n; ++t) {
+            double d = dlag64(n, t, x, x[a]);       /* l'_t(x_a) */
+            D->Dg[a][t] = (float)(d * jei);
+            D->Dw[a][t] = (float)(d * w[a]);
+        }
+    }
+}
+
+/* bilinear Jacobian of every element at every GLL point, FP64
+   (All_Je + Jacobian_4, sem2d_psvsh.c:857-889, lagrange.c:216-248);
+   records the largest deviation from eh^2/4 */
+__global__ void jacobian_kernel(int nex, int nez, int n, double eh, unsigned long long *maxdev_bits)
+{
+    double x[SEMSKS_MAXNP], w[SEMSKS_MAXNP];
+    gll64(n, x, w);
+    long nel = (long)nex * nez;
+    double dev = 0.0, je0 = eh * eh / 4.0;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nel; e += (long)gridDim.x * blockDim.x) {
+        int ex = (int)(e / nez), ez = (int)(e % nez);
+        double px[4] = {ex * eh, (ex + 1) * eh, ex * eh, (ex + 1) * eh};
+        double pz[4] = {ez * eh, ez * eh, (ez + 1) * eh, (ez + 1) * eh};
+        for (int m = 0; m <= n; ++m)
+            for (int k = 0; k <= n; ++k) {
+                double rx = x[m], rz = x[k];
+                double Nrx[4] = {-1 + rz, 1 - rz, -1 - rz, 1 + rz}, Nrz[4] = {-1 + rx, -1 - rx, 1 - rx, 1 + rx};
+                double j0 = 0, j1 = 0, j2 = 0, j3 = 0;
+                for (int i = 0; i < 4; ++i) { j0 += px[i] * Nrx[i] / 4; j1 += px[i] * Nrz[i] / 4; j2 += pz[i] * Nrx[i] / 4; j3 += pz[i] * Nrz[i] / 4; }
+                double je = j0 * j3 - j1 * j2;
+                dev = fmax(dev, fabs(je - je0));
+            }
+    }
+    atomicMax(maxdev_bits, (unsigned long long)__double_as_longlong(dev));
+}
+
+/* lumped mass M = sum_e w_m w_n Je rho in the element order of Mass_Matrix
+   (sem2d_psvsh.c:1042-1071; every term at a node is equal, so the sum is
+   k additions of one term), dt^2/M to FP32 */
+__global__ void mass_kernel(InstDesc D, const double *w64, double je, float sdt, double *mass)
+{
+    const int n = D.hpn;
+    const long N = (long)D.nx * D.nz;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < N; q += (long)gridDim.x * blockDim.x) {
+        int iz = (int)(q / D.nx), ix = (int)(q % D.nx);
+        int m = ix % n, k = iz % n;
+        int cx = 1, cz = 1;
+        if (m == 0 && ix > 0 && ix < D.nx - 1) cx = 2;
+        if (k == 0 && iz > 0 && iz < D.nz - 1) cz = 2;
+        if (ix == D.nx - 1) m = n;
+        if (iz == D.nz - 1) k = n;
+        double rho = (double)D.mat[(long)D.mat_id[(long)iz * D.ld + ix] * SEMSKS_MAT_STRIDE];
+        double t = w64[m] * w64[k] * je * rho, s = 0.0;
+        for (int c = 0; c < cx * cz; ++c) s = s + t;
+        if (mass) mass[q] = s;
+        ((float *)D.mdt2)[(long)iz * D.ld + ix] = (float)((double)sdt * (double)sdt / s);
+    }
+}
+
+__device__ static double lysmer_term(const InstDesc &D, long node, double w, double jb, float sdt, int normal, int k)
+{
+    const float *r = D.mat + (long)D.mat_id[node] * SEMSKS_MAT_STRIDE;
+    double rho = r[0];
+    double v = (k == normal) ? sqrt((double)r[1] / rho) : sqrt((double)r[10] / rho);
+    return w * rho * v * jb * sdt / 2.;
+}
+
+/* Lysmer coefficients A (Absorb_Matrix, sem2d_psvsh.c:1093-1162, summed in
+   its loop order: left, bottom, right) and the edge update factors */
+__global__ void absorb_kernel(InstDesc D, const double *w64, double jb, float sdt, int absorb, const double *mass,
+                              double *ab_out, float *ea, float *eb)
+{
+    const int n = D.hpn, nx = D.nx, nz = D.nz;
+    const int ne = semsks_n_edge(nx, nz, n);
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+        int ix, iz;
+        if (e < nx) { ix = e; iz = 0; }
+        else if (e < nx + nz - 1) { ix = 0; iz = e - nx + 1; }
+        else if (e < nx + 2 * (nz - 1)) { ix = nx - 1; iz = e - nx - (nz - 1) + 1; }
+        else { ix = e - nx - 2 * (nz - 1) + 1; iz = nz - 1; }
+        if (ix >= nx) continue;
+        long node = (long)iz * D.ld + ix;
+        double A[3] = {0, 0, 0};
+        if (absorb) {
+            for (int k = 0; k < 3; ++k) {
+                double a = 0.0;
+                if (ix == 0) {           /* left: elements ez with ez*n+be == iz, ez ascending */
+                    for (int ez = 0; ez < D.nez; ++ez) {
+                        int be = iz - ez * n;
+                        if (be >= 0 && be <= n) a = a + lysmer_term(D, node, w64[be], jb, sdt, 0, k);
+                    }
+                }
+                if (iz == 0) {           /* bottom */
+                    for (int ex = 0; ex < D.nex; ++ex) {
+                        int al = ix - ex * n;
+                        if (al >= 0 && al <= n) a = a + lysmer_term(D, node, w64[al], jb, sdt, 2, k);
+                    }
+                }
+                if (ix == nx - 1) {      /* right */
+                    for (int ez = 0; ez < D.nez; ++ez) {
+                        int be = iz - ez * n;
+                        if (be >= 0 && be <= n) a = a + lysmer_term(D, node, w64[be], jb, sdt, 0, k);
+                    }
+                }
+                A[k] = a;
+            }
+        }
+        double M = mass[(long)iz * nx + ix];
+        double dt2 = (double)sdt * (double)sdt;
+        for (int k = 0; k < 3; ++k) {
+            if (ab_out) ab_out[((long)iz * nx + ix) * 3 + k] = A[k];
+            ea[e * 3 + k] = (float)((M - A[k]) / (M + A[k]));
+            eb[e * 3 + k] = (float)(dt2 / (M + A[k]));
+        }
+    }
+}
+
+/* ========================================================================= */
+/* boundary series: 4-tap interpolation of the records + guard               */
+/* ========================================================================= */
+struct SchedEntry { int rec[4]; float w[4]; };
+
+/* G[k][i] = ((((0 + w0 s0) + w1 s1) + w2 s2) + w3 s3) in FP32 without
+   contraction (interpl.c:43-48), then the sequential non-finite / |g|>=10
+   repair of sem2d_psvsh.c:433-443 when a row needs it */
+__global__ void series_kernel(const float *rec, int width, int nrec, const SchedEntry *sched, int nlev, float *G)
+{
+    for (int k = blockIdx.x; k < nlev; k += gridDim.x) {
+        const SchedEntry s = sched[k];
+        float *g = G + (long)k * width;
+        int mybad = 0;
+        for (int i = threadIdx.x; i < width; i += blockDim.x) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float v = (s.rec[j] >= 0 && s.rec[j] < nrec) ? rec[(long)s.rec[j] * width + i] : 0.0f;
+                acc = __fadd_rn(acc, __fmul_rn(s.w[j], v));
+            }
+            g[i] = acc;
+            if (!(acc < 10.0f && acc > -10.0f)) mybad = 1;
+        }
+        mybad = __syncthreads_or(mybad);
+        if (mybad && threadIdx.x == 0) {
+            for (int i = 0; i < width; ++i)
+                if (!(g[i] < 10.0f && g[i] > -10.0f)) {
+                    float l = i > 0 ? g[i - 1] : 0.0f, r = i + 1 < width ? g[i + 1] : 0.0f;
+                    g[i] = (l + r) / 2;
+                }
+        }
+        __syncthreads();
+    }
+}
+
+/* ========================================================================= */
+/* level 0, injection, receivers                                             */
+/* ========================================================================= */
+
+/* GRT_Interface at it == 0 (sem2d_psvsh.c:1953-2005): u on the interface row
+   and column from g(0), u(-dt) = 0, so delta = u */
+__global__ void init_state_kernel(InstDesc D)
+{
+    const int nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld;
+    const float *g = D.G, *gv = D.Gv;
+    const bool sh = D.solver == SEMSKS_SOLVER_PSVSH;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nx + nz; i += gridDim.x * blockDim.x) {
+        long q;
+        float u, v = 0.f, w;
+        if (i < nx) {
+            int ix = i;
+            if (ix < nbl) continue;
+            if (ix == nbl) continue;                 /* the left pass writes the corner */
+            q = (long)nbl * ld + ix;
+            u = g[ix]; w = -g[nx + ix]; if (sh) v = gv[ix];
+        } else {
+            int iz = i - nx;
+            if (iz < nbl) continue;
+            q = (long)iz * ld + nbl;
+            u = g[2 * nx + iz]; w = -g[2 * nx + nz + iz]; if (sh) v = gv[nx + iz];
+        }
+        D.u[0][0][q] = u; D.u[0][1][q] = v; D.u[0][2][q] = w;
+        D.del[0][q] = u; D.del[1][q] = v; D.del[2][q] = w;
+    }
+}
+
+__device__ __forceinline__ void kahan_add(float &u, float &c, float x)
+{
+    float y = x - c;
+    float t = u + y;
+    c = (t - u) - y;
+    u = t;
+}
+
+/* GRT_Interface4: u += g(t) - g(t-dt) (w sign flipped); GRT_Interface3:
+   u += -u(t-dt) + g(t).  Both change u(t) in place before the force, and the
+   reference then copies the changed u(t) into u(t-dt), so delta moves with u. */
+__global__ void inject_kernel(const InstDesc *insts, const int *levels)
+{
+    const InstDesc &D = insts[blockIdx.x];
+    if (!D.active) return;
+    const int it = levels[blockIdx.x];
+    if (it >= D.nst - 1) return;
+    const int lev = it + 1;
+    if (lev >= D.nt_sem) return;
+    const int p = it & 1, nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld, mt = 2 * (nx + nz), sht = nx + nz;
+    const float *g = D.G + (long)lev * mt, *g0 = D.G + (long)(lev - 1) * mt;
+    const float *gv = D.Gv + (long)lev * sht, *gv0 = D.Gv + (long)(lev - 1) * sht;
+    const bool sh = D.solver == SEMSKS_SOLVER_PSVSH, mp = D.solver == SEMSKS_SOLVER_PSV_MP;
+    float *U[3] = {D.u[p][0], D.u[p][1], D.u[p][2]};
+    const int nb = nx - nbl, nl = nz - nbl - 1;
+    for (int i = threadIdx.x; i < nb + nl; i += blockDim.x) {
+        long q;
+        float d[3];
+        if (i < nb) {
+            int ix = nbl + i;
+            q = (long)nbl * ld + ix;
+            if (mp) {
+                d[0] = -(U[0][q] - D.del[0][q]) + g[ix];
+                d[1] = 0.f;
+                d[2] = -(U[2][q] - D.del[2][q]) - g[ix + nx];
+            } else {
+                d[0] = -g0[ix] + g[ix];
+                d[1] = sh ? -gv0[ix] + gv[ix] : 0.f;
+                d[2] = g0[ix + nx] - g[ix + nx];
+            }
+        } else {
+            int iz = nbl + 1 + (i - nb);
+            q = (long)iz * ld + nbl;
+            if (mp) {
+                d[0] = -(U[0][q] - D.del[0][q]) + g[2 * nx + iz];
+                d[1] = 0.f;
+                d[2] = -(U[2][q] - D.del[2][q]) - g[2 * nx + nz + iz];
+            } else {
+                d[0] = -g0[2 * nx + iz] + g[2 * nx + iz];
+                d[1] = sh ? -gv0[nx + iz] + gv[nx + iz] : 0.f;
+                d[2] = g0[2 * nx + nz + iz] - g[2 * nx + nz + iz];
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            float u = U[k][q];
+            if (D.kahan) {
+                float c = D.cmp[k][q];
+                kahan_add(u, c, d[k]);
+                D.cmp[k][q] = c;
+            } else {
+                u = u + d[k];
+            }
+            U[k][q] = u;
+            D.del[k][q] = D.del[k][q] + d[k];
+        }
+    }
+}
+
+/* kirrecord_zhl: u to seis.u, -w to seis.w, v to seis.v; then advance the level */
+__global__ void record_kernel(const InstDesc *insts, int *levels)
+{
+    const InstDesc &D = insts[blockIdx.x];
+    if (!D.active) return;
+    const int it = levels[blockIdx.x];
+    if (it >= D.nst - 1) return;
+    const int lev = it + 1, p = lev & 1, nt = D.ntrace;
+    float *tr = D.traces + (long)lev * 3 * nt;
+    for (int r = threadIdx.x; r < nt; r += blockDim.x) {
+        long q = D.rec[r];
+        tr[r] = D.u[p][0][q];
+        tr[nt + r] = -D.u[p][2][q];
+        tr[2 * nt + r] = D.u[p][1][q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) levels[blockIdx.x] = lev;
+}
+
+__global__ void record0_kernel(InstDesc D)
+{
+    const int nt = D.ntrace;
+    for (int r = threadIdx.x; r < nt; r += blockDim.x) {
+        long q = D.rec[r];
+        D.traces[r] = D.u[0][0][q];
+        D.traces[nt + r] = -D.u[0][2][q];
+        D.traces[2 * nt + r] = D.u[0][1][q];
+    }
+}
+
+/* ========================================================================= */
+/* the element force + update kernel                                         */
+/* ========================================================================= */
+struct MatScalar {
+    float C[15];
+};
+
+template <int N>
+__device__ __forceinline__ void load_cols(const float *__restrict__ row, int ex, bool valid, bool direct_last,
+                                          float (&v)[N + 1])
+{
+    if (valid) {
+        if constexpr (N == 4) {
+            float4 t = *reinterpret_cast<const float4 *>(row + 4 * ex);
+            v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+        } else {
+#pragma unroll
+            for (int a = 0; a < N; ++a) v[a] = row[N * ex + a];
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < N; ++a) v[a] = 0.f;
+    }
+    float nb = __shfl_down(v[0], 1, 64);
+    if (direct_last) nb = row[N * ex + N];
+    v[N] = nb;
+}
+
+template <int N>
+__device__ __forceinline__ void load_ids(const uint16_t *__restrict__ row, int ex, bool valid, bool direct_last,
+                                         int (&v)[N + 1])
+{
+#pragma unroll
+    for (int a = 0; a < N; ++a) v[a] = valid ? (int)row[N * ex + a] : 0;
+    int nb = __shfl_down(v[0], 1, 64);
+    if (direct_last) nb = row[N * ex + N];
+    v[N] = nb;
+}
+
+/* stress at one node from the 5x5 block C over {xx, zz, yz, xz, xy} and the
+   2-D strains (uy,y = ux,y = uz,y = 0, Stress_ij sem2d_psvsh.c:1345-1379) */
+__device__ __forceinline__ void stress5(const float *C, float e0, float e2, float e3, float e4, float e5,
+                                        float &s0, float &s2, float &s3, float &s4, float &s5)
+{
+    /* C: 0:00 1:02 2:03 3:04 4:05 5:22 6:23 7:24 8:25 9:33 10:34 11:35 12:44 13:45 14:55 */
+    s0 = C[0] * e0 + C[1] * e2 + C[2] * e3 + C[3] * e4 + C[4] * e5;
+    s2 = C[1] * e0 + C[5] * e2 + C[6] * e3 + C[7] * e4 + C[8] * e5;
+    s3 = C[2] * e0 + C[6] * e2 + C[9] * e3 + C[10] * e4 + C[11] * e5;
+    s4 = C[3] * e0 + C[7] * e2 + C[10] * e3 + C[12] * e4 + C[13] * e5;
+    s5 = C[4] * e0 + C[8] * e2 + C[11] * e3 + C[13] * e4 + C[14] * e5;
+}
+
+template <int N, bool UNIFORM>
+__device__ __forceinline__ void element_force(const float (&u)[N + 1][N + 1][3], float (&f)[N + 1][N + 1][3],
+                                              const float (&Dg)[N + 1][N + 1], const float (&Dw)[N + 1][N + 1],
+                                              const float (&cw)[N + 1], const float *Cu, const float *__restrict__ mat,
+                                              const int (&ids)[N + 1][N + 1])
+{
+    constexpr int NP = N + 1;
+#pragma unroll
+    for (int a = 0; a < NP; ++a)
+#pragma unroll
+        for (int b = 0; b < NP; ++b)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) f[a][b][i] = 0.f;
+#pragma unroll
+    for (int b = 0; b < NP; ++b) {
+#pragma unroll
+        for (int a = 0; a < NP; ++a) {
+            float gx0 = 0.f, gx1 = 0.f, gx2 = 0.f, gz0 = 0.f, gz1 = 0.f, gz2 = 0.f;
+#pragma unroll
+            for (int t = 0; t < NP; ++t) {
+                gx0 += Dg[a][t] * u[t][b][0];
+                gx1 += Dg[a][t] * u[t][b][1];
+                gx2 += Dg[a][t] * u[t][b][2];
+                gz0 += Dg[b][t] * u[a][t][0];
+                gz1 += Dg[b][t] * u[a][t][1];
+                gz2 += Dg[b][t] * u[a][t][2];
+            }
+            float C[15];
+            if constexpr (UNIFORM) {
+#pragma unroll
+                for (int k = 0; k < 15; ++k) C[k] = Cu[k];
+            } else {
+                const float4 *r = reinterpret_cast<const float4 *>(mat + (long)ids[a][b] * SEMSKS_MAT_STRIDE);
+                float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+                C[0] = r0.y; C[1] = r0.z; C[2] = r0.w; C[3] = r1.x; C[4] = r1.y; C[5] = r1.z; C[6] = r1.w;
+                C[7] = r2.x; C[8] = r2.y; C[9] = r2.z; C[10] = r2.w; C[11] = r3.x; C[12] = r3.y; C[13] = r3.z;
+                C[14] = r3.w;
+            }
+            float s0, s2, s3, s4, s5;
+            stress5(C, gx0, gz2, gz1, gx2 + gz0, gx1, s0, s2, s3, s4, s5);
+            const float cb = cw[b], ca = cw[a];
+            const float p0 = cb * s0, p1 = cb * s5, p2 = cb * s4;   /* row traction   (T_i0) */
+            const float q0 = ca * s4, q1 = ca * s3, q2 = ca * s2;   /* column traction (T_i2) */
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                f[k][b][0] += Dw[a][k] * p0;
+                f[k][b][1] += Dw[a][k] * p1;
+                f[k][b][2] += Dw[a][k] * p2;
+                f[a][k][0] += Dw[b][k] * q0;
+                f[a][k][1] += Dw[b][k] * q1;
+                f[a][k][2] += Dw[b][k] * q2;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int edge_index(int ix, int iz, int nx, int nz, int n)
+{
+    if (iz == 0) return ix;
+    if (ix == 0) return nx + iz - 1;
+    if (ix == nx - 1) return nx + (nz - 1) + iz - 1;
+    if (iz == nz - 1 && ix >= 1 && ix <= 3 * n) return nx + 2 * (nz - 1) + ix - 1;
+    return -1;
+}
+
+/* update one owned node from its assembled force f and its u(t) */
+template <bool FORCE_ONLY>
+__device__ __forceinline__ void update_node(const InstDesc &D, int p, long q, int ix, int iz, const float (&f)[3],
+                                            const float (&uc)[3])
+{
+    if constexpr (FORCE_ONLY) {
+        D.fout[0][q] = f[0]; D.fout[1][q] = f[1]; D.fout[2][q] = f[2];
+        return;
+    } else {
+        const int e = edge_index(ix, iz, D.nx, D.nz, D.hpn);
+        const float m = D.mdt2[q];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float d = D.del[k][q];
+            if (e < 0) d = d - m * f[k];
+            else d = D.edge_a[e * 3 + k] * d - D.edge_b[e * 3 + k] * f[k];
+            float un = uc[k];
+            if (D.kahan) {
+                float c = D.cmp[k][q];
+                kahan_add(un, c, d);
+                D.cmp[k][q] = c;
+            } else {
+                un = un + d;
+            }
+            D.del[k][q] = d;
+            D.u[p ^ 1][k][q] = un;
+        }
+    }
+}
+
+/* four consecutive interior-row nodes at once (N == 4 vector path) */
+template <bool FORCE_ONLY>
+__device__ __forceinline__ void update_row4(const InstDesc &D, int p, long q0, int ix0, int iz,
+                                            const float (&f)[5][5][3], const float (&u)[5][5][3], int b)
+{
+    if constexpr (FORCE_ONLY) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            *reinterpret_cast<float4 *>(D.fout[k] + q0) = make_float4(f[0][b][k], f[1][b][k], f[2][b][k], f[3][b][k]);
+        return;
+    } else {
+        const bool edge = (iz == 0) || (ix0 == 0) || (iz == D.nz - 1 && ix0 <= 3 * D.hpn);
+        if (edge) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                float fa[3] = {f[a][b][0], f[a][b][1], f[a][b][2]};
+                float ua[3] = {u[a][b][0], u[a][b][1], u[a][b][2]};
+                update_node<false>(D, p, q0 + a, ix0 + a, iz, fa, ua);
+            }
+            return;
+        }
+        const float4 m = *reinterpret_cast<const float4 *>(D.mdt2 + q0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float4 d = *reinterpret_cast<const float4 *>(D.del[k] + q0);
+            d.x = d.x - m.x * f[0][b][k];
+            d.y = d.y - m.y * f[1][b][k];
+            d.z = d.z - m.z * f[2][b][k];
+            d.w = d.w - m.w * f[3][b][k];
+            float4 un = make_float4(u[0][b][k], u[1][b][k], u[2][b][k], u[3][b][k]);
+            if (D.kahan) {
+                float4 c = *reinterpret_cast<const float4 *>(D.cmp[k] + q0);
+                kahan_add(un.x, c.x, d.x);
+                kahan_add(un.y, c.y, d.y);
+                kahan_add(un.z, c.z, d.z);
+                kahan_add(un.w, c.w, d.w);
+                *reinterpret_cast<float4 *>(D.cmp[k] + q0) = c;
+            } else {
+                un.x += d.x; un.y += d.y; un.z += d.z; un.w += d.w;
+            }
+            *reinterpret_cast<float4 *>(D.del[k] + q0) = d;
+            *reinterpret_cast<float4 *>(D.u[p ^ 1][k] + q0) = un;
+        }
+    }
+}
+
+template <int N, bool FORCE_ONLY>
+__device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows, int lane)
+{
+    constexpr int NP = N + 1;
+    const int nex = D.nex, nez = D.nez, ld = D.ld;
+    const int ex = x0 - 1 + lane;
+    const bool valid = ex >= 0 && ex < nex;
+    const bool own_x = valid && lane >= 1;
+    const bool last_x = ex == nex - 1;
+    const bool direct = valid && (lane == 63 || last_x);
+
+    float Dg[NP][NP], Dw[NP][NP], cw[NP];
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+        cw[a] = D.cw[a];
+#pragma unroll
+        for (int t = 0; t < NP; ++t) { Dg[a][t] = D.Dg[a][t]; Dw[a][t] = D.Dw[a][t]; }
+    }
+    const float *__restrict__ U0 = FORCE_ONLY ? D.u[0][0] : D.u[p][0];
+    const float *__restrict__ U1 = FORCE_ONLY ? D.u[0][1] : D.u[p][1];
+    const float *__restrict__ U2 = FORCE_ONLY ? D.u[0][2] : D.u[p][2];
+
+    float u[NP][NP][3];
+    float f[NP][NP][3];
+    float cf[NP][3];
+    int ids[NP][NP];
+#pragma unroll
+    for (int a = 0; a < NP; ++a)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) cf[a][i] = 0.f;
+
+    const int ez0 = z0 > 0 ? z0 - 1 : 0;
+    const int ez1 = z0 + rows;
+    for (int ez = ez0; ez < ez1; ++ez) {
+        const bool first = ez == ez0;
+        /* rows of u: row 0 comes from the previous element's top row */
+#pragma unroll
+        for (int b = 0; b < NP; ++b) {
+            if (b == 0 && !first) {
+#pragma unroll
+                for (int a = 0; a < NP; ++a)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) u[a][0][i] = u[a][N][i];
+                continue;
+            }
+            const long rowoff = (long)(ez * N + b) * ld;
+            float v0[NP], v1[NP], v2[NP];
+            load_cols<N>(U0 + rowoff, ex, valid, direct, v0);
+            load_cols<N>(U1 + rowoff, ex, valid, direct, v1);
+            load_cols<N>(U2 + rowoff, ex, valid, direct, v2);
+#pragma unroll
+            for (int a = 0; a < NP; ++a) { u[a][b][0] = v0[a]; u[a][b][1] = v1[a]; u[a][b][2] = v2[a]; }
+        }
+        /* material: one record for the whole wave when every element in it
+           is uniform with the same record, else per node */
+        const int em = valid ? (int)D.emat[(long)ez * nex + ex] : -1;
+        int ref;
+        {
+            const unsigned long long vm = __ballot(valid);
+            const int src = vm ? __ffsll((long long)vm) - 1 : 0;
+            ref = __shfl(em, src, 64);
+        }
+        const bool uniform = __all(!valid || em == ref) && ref != 0xFFFF && ref >= 0;
+        if (uniform) {
+            const int rid = __builtin_amdgcn_readfirstlane(ref);
+            const float *r = D.mat + (long)rid * SEMSKS_MAT_STRIDE;
+            float C[15];
+#pragma unroll
+            for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
+            element_force<N, true>(u, f, Dg, Dw, cw, C, D.mat, ids);
+        } else {
+#pragma unroll
+            for (int b = 0; b < NP; ++b) {
+                int v[NP];
+                load_ids<N>(D.mat_id + (long)(ez * N + b) * ld, ex, valid, direct, v);
+#pragma unroll
+                for (int a = 0; a < NP; ++a) ids[a][b] = v[a];
+            }
+            element_force<N, false>(u, f, Dg, Dw, cw, nullptr, D.mat, ids);
+        }
+        if (!valid) {
+#pragma unroll
+            for (int a = 0; a < NP; ++a)
+#pragma unroll
+                for (int b = 0; b < NP; ++b)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) f[a][b][i] = 0.f;
+        }
+        /* assemble: bottom row gets the carried top row of the element below */
+#pragma unroll
+        for (int a = 0; a < NP; ++a)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                f[a][0][i] += cf[a][i];
+                cf[a][i] = f[a][N][i];
+            }
+        const bool top = ez == nez - 1;
+        /* left column gets the right column of the lane to the left */
+#pragma unroll
+        for (int b = 0; b < NP; ++b) {
+            if (b == N && !top) continue;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                float s = __shfl_up(f[N][b][i], 1, 64);
+                if (lane > 0) f[0][b][i] += s;
+            }
+        }
+        if (own_x && ez >= z0) {
+#pragma unroll
+            for (int b = 0; b < NP; ++b) {
+                if (b == N && !top) continue;
+                const int iz = ez * N + b;
+                const long rowoff = (long)iz * ld;
+                const int ixb = ex * N;
+                if constexpr (N == 4) {
+                    update_row4<FORCE_ONLY>(D, p, rowoff + ixb, ixb, iz, f, u, b);
+                } else {
+#pragma unroll
+                    for (int a = 0; a < N; ++a) {
+                        float fa[3] = {f[a][b][0], f[a][b][1], f[a][b][2]};
+                        float ua[3] = {u[a][b][0], u[a][b][1], u[a][b][2]};
+                        update_node<FORCE_ONLY>(D, p, rowoff + ixb + a, ixb + a, iz, fa, ua);
+                    }
+                }
+                if (last_x) {
+                    float fa[3] = {f[N][b][0], f[N][b][1], f[N][b][2]};
+                    float ua[3] = {u[N][b][0], u[N][b][1], u[N][b][2]};
+                    update_node<FORCE_ONLY>(D, p, rowoff + ixb + N, ixb + N, iz, fa, ua);
+                }
+            }
+        }
+    }
+}
+
+/* persistent grid: each 64-lane wave takes tiles (slot, x0, z0, rows) from the
+   work list until it is exhausted */
+template <int N, bool FORCE_ONLY>
+__global__ __launch_bounds__(256) void step_kernel(const InstDesc *__restrict__ insts, const int *__restrict__ levels,
+                                                   const int4 *__restrict__ tiles, const int *__restrict__ ntiles)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int nt = *ntiles;
+    for (int t = wave; t < nt; t += nwaves) {
+        int4 tile = tiles[t];
+        tile.x = __builtin_amdgcn_readfirstlane(tile.x);
+        tile.y = __builtin_amdgcn_readfirstlane(tile.y);
+        tile.z = __builtin_amdgcn_readfirstlane(tile.z);
+        tile.w = __builtin_amdgcn_readfirstlane(tile.w);
+        const InstDesc &D = insts[tile.x];
+        if (!D.active || D.hpn != N) continue;
+        int p = 0;
+        if constexpr (!FORCE_ONLY) {
+            const int it = levels[tile.x];
+            if (it >= D.nst - 1) continue;
+            p = it & 1;
+        }
+        process_tile<N, FORCE_ONLY>(D, p, tile.y, tile.z, tile.w, lane);
+    }
+}
+
+/* ========================================================================= */
+/* closed-form plane wave records (stand-in for the GRT producer)            */
+/* ========================================================================= */
+__device__ static double node_coord(int i, int nel, double eh, int n, const double *xi)
+{
+    int e = i / n;
+    if (e > nel - 1) e = nel - 1;
+    int a = i - e * n;
+    return e * eh + (xi[a] + 1.0) / 2.0 * eh;
+}
+
+__global__ void planewave_kernel(int nex, int nez, int n, int neh, double eh, int nt, double dp, double p, double beta,
+                                 double f0, double psi, float *psv, float *sh)
+{
+    double xi[SEMSKS_MAXNP], w[SEMSKS_MAXNP];
+    gll64(n, xi, w);
+    const int nx = nex * n + 1, nz = nez * n + 1, nbl = neh * n;
+    const int mt = 2 * (nx + nz), sht = nx + nz;
+    const double s = p * beta, ci = sqrt(1.0 - s * s), eta = ci / beta, lead = 1.2 / f0;
+    const double pf = (M_PI * f0) * (M_PI * f0), cp = cos(psi * (M_PI / 180.0)), sp = sin(psi * (M_PI / 180.0));
+    const long total = (long)nt * sht;
+    for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < total; id += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(id / sht), j = (int)(id % sht);
+        double x, z;
+        if (j < nx) { x = node_coord(j, nex, eh, n, xi); z = node_coord(nbl, nez, eh, n, xi); }
+        else { x = node_coord(nbl, nex, eh, n, xi); z = node_coord(j - nx, nez, eh, n, xi); }
+        const double t = k * dp;
+        const double tau = t - lead - (p * x + eta * z);
+        const double a = pf * tau * tau;
+        const double r = (1.0 - 2.0 * a) * exp(-a);
+        const double sv = r * cp;
+        const float uu = (float)(sv * ci), wd = (float)(-(-sv * s)), vv = (float)(r * sp);
+        float *rp = psv + (long)k * mt;
+        if (j < nx) { rp[j] = uu; rp[nx + j] = wd; }
+        else { rp[2 * nx + (j - nx)] = uu; rp[2 * nx + nz + (j - nx)] = wd; }
+        sh[(long)k * sht + j] = vv;
+    }
+}
+
+/* ========================================================================= */
+/* engine                                                                    */
+/* ========================================================================= */
+namespace {
+
+struct Slot {
+    bool used = false;
+    int share = -1;           /* slot whose mesh/material this one uses */
+    int refs = 0;             /* slots sharing this one's material */
+    semsks_problem meta{};
+    InstDesc d{};
+    int nx = 0, nz = 0, ld = 0, nt_sem = 0, nlev = 0;
+    size_t bytes = 0;
+    std::vector<void *> allocs;       /* owned device allocations */
+};
+
+template <class T>
+hipError_t dalloc(Slot &s, T **p, size_t n)
+{
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T) + 256);
+    if (e == hipSuccess) { s.allocs.push_back(q); s.bytes += n * sizeof(T); }
+    *p = (T *)q;
+    return e;
+}
+
+}  // namespace
+
+struct semsks_engine {
+    semsks_engine_opts opts{};
+    hipStream_t stream = nullptr;
+    int device = 0, n_cu = 0;
+    size_t budget = 0, used = 0;
+    std::vector<Slot> slots;
+    InstDesc *d_desc = nullptr;
+    int *d_levels = nullptr;
+    int4 *d_tiles = nullptr;
+    int *d_ntiles = nullptr;          /* [7]: per hpn 3..6 (index hpn) */
+    std::vector<int> h_levels;
+    int tile_cap = 0;
+    int tile_off[SEMSKS_MAXNP] = {0}; /* offset of each hpn's tiles */
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_steps = 0;
+    unsigned hpn_mask = 0, graph_mask = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> tev;      /* per-step timing events */
+    int timed_steps = 0;
+    double last_ms[2] = {0, 0};
+    /* device-resident plane-wave records waiting for the next admit */
+    float *pend_psv = nullptr, *pend_sh = nullptr;
+    int pend_nt = 0, pend_nx = 0, pend_nz = 0;
+};
+
+static int tile_rows(const semsks_engine *e, int nez)
+{
+    int tz = e->opts.tile_z > 0 ? e->opts.tile_z : 8;
+    return std::min(tz, std::max(1, nez));
+}
+
+static semsks_status upload_tables(semsks_engine *e)
+{
+    /* descriptors */
+    std::vector<InstDesc> h(e->slots.size());
+    for (size_t i = 0; i < e->slots.size(); ++i) {
+        h[i] = e->slots[i].d;
+        h[i].active = e->slots[i].used ? 1 : 0;
+    }
+    HIPCHK(hipMemcpyAsync(e->d_desc, h.data(), h.size() * sizeof(InstDesc), hipMemcpyHostToDevice, e->stream));
+    /* tiles, grouped by hpn, slots interleaved per spatial tile so that
+       concurrent waves touch the same region of a shared mesh */
+    std::vector<int4> t;
+    int cnt[SEMSKS_MAXNP] = {0};
+    e->hpn_mask = 0;
+    for (int n = 3; n <= 6; ++n) {
+        e->tile_off[n] = (int)t.size();
+        int maxx = 0, maxz = 0;
+        for (auto &s : e->slots)
+            if (s.used && s.meta.hpn == n) { maxx = std::max(maxx, s.meta.nex); maxz = std::max(maxz, s.meta.nez); }
+        const int tzg = e->opts.tile_z > 0 ? e->opts.tile_z : 8;
+        for (int z0 = 0; z0 < maxz; z0 += tzg) {
+            for (int x0 = 0; x0 < maxx; x0 += SEMSKS_XCHUNK) {
+                for (size_t k = 0; k < e->slots.size(); ++k) {
+                    const Slot &s = e->slots[k];
+                    if (!s.used || s.meta.hpn != n) continue;
+                    if (z0 >= s.meta.nez || x0 >= s.meta.nex) continue;
+                    t.push_back(make_int4((int)k, x0, z0, std::min(tzg, s.meta.nez - z0)));
+                }
+            }
+        }
+        cnt[n] = (int)t.size() - e->tile_off[n];
+        if (cnt[n]) e->hpn_mask |= 1u << n;
+    }
+    if ((int)t.size() > e->tile_cap) {
+        if (e->d_tiles) HIPCHK(hipFree(e->d_tiles));
+        e->tile_cap = (int)t.size() + 1024;
+        HIPCHK(hipMalloc(&e->d_tiles, e->tile_cap * sizeof(int4)));
+    }
+    if (!t.empty())
+        HIPCHK(hipMemcpyAsync(e->d_tiles, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->d_ntiles, cnt, sizeof cnt, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->dirty = false;
+    return SEMSKS_OK;
+}
+
+static int persistent_blocks(const semsks_engine *e)
+{
+    int wpc = e->opts.waves_per_cu > 0 ? e->opts.waves_per_cu : 8;
+    return std::max(1, e->n_cu * wpc / 4);
+}
+
+static semsks_status launch_step(semsks_engine *e, hipEvent_t ta, hipEvent_t tb)
+{
+    const int nslots = (int)e->slots.size();
+    hipLaunchKernelGGL(inject_kernel, dim3(nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
+    if (ta) HIPCHK(hipEventRecord(ta, e->stream));
+    const int nb = persistent_blocks(e);
+    for (int n = 3; n <= 6; ++n) {
+        if (!(e->hpn_mask & (1u << n))) continue;
+        const int4 *tl = e->d_tiles + e->tile_off[n];
+        const int *nt = e->d_ntiles + n;
+        switch (n) {
+        case 3: hipLaunchKernelGGL((step_kernel<3, false>), dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels, tl, nt); break;
+        case 4: hipLaunchKernelGGL((step_kernel<4, false>), dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels, tl, nt); break;
+        case 5: hipLaunchKernelGGL((step_kernel<5, false>), dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels, tl, nt); break;
+        case 6: hipLaunchKernelGGL((step_kernel<6, false>), dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels, tl, nt); break;
+        }
+    }
+    if (tb) HIPCHK(hipEventRecord(tb, e->stream));
+    hipLaunchKernelGGL(record_kernel, dim3(nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
+    HIPCHK(hipGetLastError());
+    return SEMSKS_OK;
+}
+
+static void drop_graph(semsks_engine *e)
+{
+    if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+    if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
+    e->graph_steps = 0;
+}
+
+static semsks_status build_graph(semsks_engine *e, int steps)
+{
+    drop_graph(e);
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < steps; ++s) {
+        semsks_status st = launch_step(e, nullptr, nullptr);
+        if (st != SEMSKS_OK) { hipGraph_t g; hipStreamEndCapture(e->stream, &g); if (g) hipGraphDestroy(g); return st; }
+    }
+    HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
+    HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+    e->graph_steps = steps;
+    e->graph_mask = e->hpn_mask;
+    return SEMSKS_OK;
+}
+
+extern "C" {
+
+semsks_status semsks_engine_create(const semsks_engine_opts *opts, semsks_engine **out)
+{
+    *out = nullptr;
+    semsks_engine *e = new semsks_engine();
+    if (opts) e->opts = *opts;
+    if (e->opts.max_slots <= 0) e->opts.max_slots = 1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete e;
+        return semsks_set_error(SEMSKS_EDEVICE, "no HIP device");
+    }
+    HIPCHK(hipGetDevice(&e->device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, e->device));
+    e->n_cu = prop.multiProcessorCount;
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    e->budget = e->opts.hbm_budget ? e->opts.hbm_budget : (size_t)(0.9 * (double)fr);
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    for (auto &v : e->ev) HIPCHK(hipEventCreate(&v));
+    e->slots.resize(e->opts.max_slots);
+    e->h_levels.assign(e->opts.max_slots, -1);
+    HIPCHK(hipMalloc(&e->d_desc, e->slots.size() * sizeof(InstDesc)));
+    HIPCHK(hipMalloc(&e->d_levels, e->slots.size() * sizeof(int)));
+    HIPCHK(hipMalloc(&e->d_ntiles, SEMSKS_MAXNP * sizeof(int)));
+    std::vector<int> z(e->slots.size(), 0);
+    HIPCHK(hipMemcpy(e->d_levels, z.data(), z.size() * sizeof(int), hipMemcpyHostToDevice));
+    semsks_status st = upload_tables(e);
+    if (st != SEMSKS_OK) { semsks_engine_destroy(e); return st; }
+    *out = e;
+    return SEMSKS_OK;
+}
+
+static void free_slot(semsks_engine *e, int k)
+{
+    Slot &s = e->slots[k];
+    for (void *p : s.allocs) hipFree(p);
+    e->used -= s.bytes;
+    if (s.share >= 0) e->slots[s.share].refs--;
+    s = Slot();
+}
+
+void semsks_engine_destroy(semsks_engine *e)
+{
+    if (!e) return;
+    if (e->stream) hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    for (size_t k = 0; k < e->slots.size(); ++k) if (e->slots[k].used) free_slot(e, (int)k);
+    if (e->pend_psv) hipFree(e->pend_psv);
+    if (e->pend_sh) hipFree(e->pend_sh);
+    hipFree(e->d_desc); hipFree(e->d_levels); hipFree(e->d_tiles); hipFree(e->d_ntiles);
+    for (auto &v : e->ev) if (v) hipEventDestroy(v);
+    for (auto &v : e->tev) hipEventDestroy(v);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+static int nx_of(const semsks_problem *p) { return p->nex * p->hpn + 1; }
+static int nz_of(const semsks_problem *p) { return p->nez * p->hpn + 1; }
+static int ld_of(const semsks_problem *p) { return ((nx_of(p) + 3 + 63) / 64) * 64; }
+static int nt_sem_of(const semsks_problem *p) { return (int)((float)((p->nt - 3) * p->dp / p->sdt)); }
+
+size_t semsks_footprint(const semsks_problem *p, int shared)
+{
+    const size_t nodes = (size_t)ld_of(p) * nz_of(p);
+    const int kahan = !(p->flags & SEMSKS_F_NO_KAHAN);
+    size_t b = nodes * 4 * (6 + 3 + (kahan ? 3 : 0));                  /* u x2, delta, comp */
+    if (!shared) b += nodes * (2 + 4) + (size_t)p->nex * p->nez * 2 + (size_t)p->n_mat * 64 +
+                      (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24;
+    const int mt = 2 * (nx_of(p) + nz_of(p)), sht = nx_of(p) + nz_of(p);
+    int nlev = std::min(std::max(nt_sem_of(p), 1), std::max(p->nst, 1));
+    b += (size_t)nlev * (mt + sht) * 4;
+    b += (size_t)p->nst * 3 * std::max(p->ntrace, 1) * 4 + (size_t)p->ntrace * 8;
+    return b;
+}
+
+/* host mirror of interp_const's slot ring (interpl.c:59-91) for one or two
+   streams sharing it: which record sits in each tap and its weight */
+static void build_schedule(int nlev, float rdt, int nstream, const int nrec[2], std::vector<SchedEntry> out[2])
+{
+    int ns[4] = {0, 1, 2, 3};
+    int slot_rec[2][4] = {{-1, -1, -1, -1}, {-1, -1, -1, -1}};
+    int nread[2] = {0, 0};
+    out[0].resize(nlev);
+    out[1].resize(nlev);
+    for (int it = 0; it < nlev; ++it) {
+        for (int s = 0; s < nstream; ++s) {
+            const float nst0 = (float)it, t = nst0 * rdt;
+            const int ns0 = (int)(t + 0.00001);
+            if (ns0 > (int)((nst0 - 1) * rdt + 0.00001) || nst0 == 0.) {
+                int r = ns[0];
+                for (int j = 0; j < 3; ++j) ns[j] = ns[j + 1];
+                ns[3] = r;
+                slot_rec[s][ns[3]] = nread[s] < nrec[s] ? nread[s] : -1;
+                nread[s]++;
+            }
+            const float d1 = t - (int)(t + 0.00001);
+            const float dty1 = d1, dty2 = dty1 * dty1, dty3 = dty1 * dty2;
+            SchedEntry &E = out[s][it];
+            E.w[0] = (float)(-0.5 * dty3 + dty2 - 0.5 * dty1);
+            E.w[1] = (float)(3.0 * dty3 - 2.5 * dty2 + 1.0);
+            E.w[2] = (float)(-3.0 * dty3 + 2.0 * dty2 + 0.5 * dty1);
+            E.w[3] = (float)(0.5 * (dty3 - dty2));
+            for (int j = 0; j < 4; ++j) E.rec[j] = slot_rec[s][ns[j]];
+        }
+    }
+}
+
+semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share, int *slot_out)
+{
+    if (!e || !p || !slot_out) return semsks_set_error(SEMSKS_EINVAL, "null argument");
+    if (p->hpn < 3 || p->hpn > 6 || p->nex <= p->neh || p->nez <= p->neh || p->nst < 1 || !(p->sdt > 0) ||
+        !(p->eh > 0) || p->n_mat < 1 || !p->mat_table || !p->mat_id || p->ntrace < 0 || p->nt < 0 ||
+        (p->solver < 0 || p->solver > 2))
+        return semsks_set_error(SEMSKS_EINVAL, "invalid problem description");
+    int k = -1;
+    for (size_t i = 0; i < e->slots.size(); ++i) if (!e->slots[i].used) { k = (int)i; break; }
+    if (k < 0) return semsks_set_error(SEMSKS_ENOSLOT, "all %zu slots busy", e->slots.size());
+    if (share >= 0) {
+        if (share >= (int)e->slots.size() || !e->slots[share].used || e->slots[share].share >= 0)
+            return semsks_set_error(SEMSKS_EINVAL, "share_slot %d is not a material owner", share);
+        const semsks_problem &q = e->slots[share].meta;
+        if (q.nex != p->nex || q.nez != p->nez || q.hpn != p->hpn || q.eh != p->eh || q.sdt != p->sdt ||
+            q.absorb != p->absorb)
+            return semsks_set_error(SEMSKS_EINVAL, "share_slot %d has a different mesh", share);
+    }
+    const size_t need = semsks_footprint(p, share >= 0);
+    if (e->used + need > e->budget)
+        return semsks_set_error(SEMSKS_ENOHBM, "slot needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
+    for (int i = 0; i < p->ntrace; ++i)
+        if (p->rec_ix[i] < 0 || p->rec_ix[i] >= nx_of(p) || p->rec_iz[i] < 0 || p->rec_iz[i] >= nz_of(p))
+            return semsks_set_error(SEMSKS_EINVAL, "receiver %d outside the mesh", i);
+    for (long q = 0; q < (long)nx_of(p) * nz_of(p); ++q)
+        if (p->mat_id[q] >= p->n_mat) return semsks_set_error(SEMSKS_EINVAL, "material id out of range at node %ld", q);
+
+    Slot &s = e->slots[k];
+    s = Slot();
+    s.meta = *p;
+    s.nx = nx_of(p); s.nz = nz_of(p); s.ld = ld_of(p);
+    s.nt_sem = nt_sem_of(p);
+    s.nlev = std::max(1, std::min(s.nt_sem, p->nst));
+    s.share = share;
+    InstDesc &d = s.d;
+    memset(&d, 0, sizeof d);
+    d.nex = p->nex; d.nez = p->nez; d.nx = s.nx; d.nz = s.nz; d.ld = s.ld;
+    d.hpn = p->hpn; d.neh = p->neh; d.nst = p->nst; d.nt_sem = s.nt_sem; d.solver = p->solver;
+    d.kahan = (p->flags & SEMSKS_F_NO_KAHAN) ? 0 : 1;
+    d.ntrace = p->ntrace;
+    const size_t nodes = (size_t)s.ld * s.nz;
+    const int mt = 2 * (s.nx + s.nz), sht = s.nx + s.nz;
+    semsks_status st = SEMSKS_OK;
+#define ACHK(x) do { if ((x) != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "device allocation failed"); goto fail; } } while (0)
+    {
+        float *st_all = nullptr;
+        const int nplanes = 6 + 3 + (d.kahan ? 3 : 0);
+        ACHK(dalloc(s, &st_all, nodes * nplanes));
+        ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->stream));
+        for (int b = 0; b < 2; ++b) for (int c = 0; c < 3; ++c) d.u[b][c] = st_all + (b * 3 + c) * nodes;
+        for (int c = 0; c < 3; ++c) d.del[c] = st_all + (6 + c) * nodes;
+        for (int c = 0; c < 3; ++c) d.cmp[c] = d.kahan ? st_all + (9 + c) * nodes : nullptr;
+    }
+    if (share >= 0) {
+        const InstDesc &o = e->slots[share].d;
+        d.mat_id = o.mat_id; d.emat = o.emat; d.mat = o.mat; d.mdt2 = o.mdt2; d.edge_a = o.edge_a; d.edge_b = o.edge_b;
+        memcpy(d.Dg, o.Dg, sizeof d.Dg); memcpy(d.Dw, o.Dw, sizeof d.Dw); memcpy(d.cw, o.cw, sizeof d.cw);
+        e->slots[share].refs++;
+    } else {
+        uint16_t *mid = nullptr, *emat = nullptr;
+        float *mat = nullptr, *mdt2 = nullptr, *ea = nullptr, *eb = nullptr;
+        ACHK(dalloc(s, &mid, nodes));
+        ACHK(dalloc(s, &emat, (size_t)p->nex * p->nez));
+        ACHK(dalloc(s, &mat, (size_t)p->n_mat * SEMSKS_MAT_STRIDE));
+        ACHK(dalloc(s, &mdt2, nodes));
+        const int ne = semsks_n_edge(s.nx, s.nz, p->hpn);
+        ACHK(dalloc(s, &ea, (size_t)ne * 3));
+        ACHK(dalloc(s, &eb, (size_t)ne * 3));
+        ACHK(hipMemsetAsync(mid, 0, nodes * 2, e->stream));
+        ACHK(hipMemsetAsync(mdt2, 0, nodes * 4, e->stream));
+        ACHK(hipMemcpy2DAsync(mid, s.ld * 2, p->mat_id, s.nx * 2, s.nx * 2, s.nz, hipMemcpyHostToDevice, e->stream));
+        ACHK(hipMemcpyAsync(mat, p->mat_table, (size_t)p->n_mat * SEMSKS_MAT_STRIDE * 4, hipMemcpyHostToDevice, e->stream));
+        {   /* element-uniform record ids */
+            std::vector<uint16_t> em((size_t)p->nex * p->nez);
+            const int n = p->hpn;
+            for (int ez = 0; ez < p->nez; ++ez)
+                for (int ex = 0; ex < p->nex; ++ex) {
+                    uint16_t r = p->mat_id[(size_t)(ez * n) * s.nx + ex * n];
+                    for (int b = 0; b <= n && r != 0xFFFF; ++b)
+                        for (int a = 0; a <= n; ++a)
+                            if (p->mat_id[(size_t)(ez * n + b) * s.nx + ex * n + a] != r) { r = 0xFFFF; break; }
+                    em[(size_t)ez * p->nex + ex] = r;
+                }
+            ACHK(hipMemcpyAsync(emat, em.data(), em.size() * 2, hipMemcpyHostToDevice, e->stream));
+            ACHK(hipStreamSynchronize(e->stream));
+        }
+        d.mat_id = mid; d.emat = emat; d.mat = mat; d.mdt2 = mdt2; d.edge_a = ea; d.edge_b = eb;
+        /* FP64 setup on the device */
+        {
+            InstDesc *dd = nullptr;
+            double *w64 = nullptr, *mass = nullptr;
+            unsigned long long *dev = nullptr;
+            ACHK(hipMalloc(&dd, sizeof(InstDesc)));
+            ACHK(hipMalloc(&w64, SEMSKS_MAXNP * sizeof(double)));
+            ACHK(hipMalloc(&mass, (size_t)s.nx * s.nz * sizeof(double)));
+            ACHK(hipMalloc(&dev, sizeof(unsigned long long)));
+            ACHK(hipMemsetAsync(dev, 0, sizeof(unsigned long long), e->stream));
+            ACHK(hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, e->stream));
+            hipLaunchKernelGGL(setup_tables_kernel, dim3(1), dim3(64), 0, e->stream, dd, w64, (double)p->eh);
+            hipLaunchKernelGGL(jacobian_kernel, dim3(256), dim3(256), 0, e->stream, p->nex, p->nez, p->hpn, (double)p->eh, dev);
+            const double eh = p->eh, je = (double)((double)eh * eh / 4.0), jb = eh / 2.0;
+            hipLaunchKernelGGL(mass_kernel, dim3(1024), dim3(256), 0, e->stream, d, w64, je, p->sdt, mass);
+            hipLaunchKernelGGL(absorb_kernel, dim3(64), dim3(256), 0, e->stream, d, w64, jb, p->sdt, p->absorb, mass,
+                               (double *)nullptr, ea, eb);
+            InstDesc back;
+            ACHK(hipMemcpyAsync(&back, dd, sizeof back, hipMemcpyDeviceToHost, e->stream));
+            unsigned long long devbits = 0;
+            ACHK(hipMemcpyAsync(&devbits, dev, sizeof devbits, hipMemcpyDeviceToHost, e->stream));
+            ACHK(hipStreamSynchronize(e->stream));
+            memcpy(d.Dg, back.Dg, sizeof d.Dg); memcpy(d.Dw, back.Dw, sizeof d.Dw); memcpy(d.cw, back.cw, sizeof d.cw);
+            double maxdev;
+            memcpy(&maxdev, &devbits, sizeof maxdev);
+            hipFree(dd); hipFree(w64); hipFree(mass); hipFree(dev);
+            if (!(maxdev <= 1.e-2)) { st = semsks_set_error(SEMSKS_EINVAL, "non-uniform Jacobian (max dev %g)", maxdev); goto fail; }
+        }
+    }
+    /* boundary series */
+    {
+        float *G = nullptr, *Gv = nullptr;
+        ACHK(dalloc(s, &G, (size_t)s.nlev * mt));
+        ACHK(dalloc(s, &Gv, (size_t)s.nlev * sht));
+        ACHK(hipMemsetAsync(G, 0, (size_t)s.nlev * mt * 4, e->stream));
+        ACHK(hipMemsetAsync(Gv, 0, (size_t)s.nlev * sht * 4, e->stream));
+        const float rdt = p->sdt / p->dp;
+        const int nstream = p->solver == SEMSKS_SOLVER_PSVSH ? 2 : 1;
+        const bool pending = !p->psv && !p->sh && e->pend_psv && e->pend_nt == p->nt && e->pend_nx == s.nx &&
+                             e->pend_nz == s.nz;
+        int nrec[2] = {(p->psv || pending) ? p->nt : 0, (p->sh || pending) ? p->nt : 0};
+        std::vector<SchedEntry> sch[2];
+        const int nlev_series = std::min(s.nt_sem, s.nlev);
+        if (nlev_series > 0) {
+            build_schedule(nlev_series, rdt, nstream, nrec, sch);
+            SchedEntry *dsch = nullptr;
+            ACHK(hipMalloc(&dsch, nlev_series * sizeof(SchedEntry)));
+            for (int str = 0; str < nstream; ++str) {
+                const int width = str ? sht : mt;
+                const float *src = str ? p->sh : p->psv;
+                float *drec = nullptr;
+                bool own = false;
+                if (pending) drec = str ? e->pend_sh : e->pend_psv;
+                else if (src && p->nt > 0) {
+                    ACHK(hipMalloc(&drec, (size_t)p->nt * width * 4));
+                    ACHK(hipMemcpyAsync(drec, src, (size_t)p->nt * width * 4, hipMemcpyHostToDevice, e->stream));
+                    own = true;
+                }
+                if (!drec) continue;
+                ACHK(hipMemcpyAsync(dsch, sch[str].data(), nlev_series * sizeof(SchedEntry), hipMemcpyHostToDevice, e->stream));
+                hipLaunchKernelGGL(series_kernel, dim3(std::min(nlev_series, 65535)), dim3(256), 0, e->stream, drec,
+                                   width, nrec[str], dsch, nlev_series, str ? Gv : G);
+                ACHK(hipStreamSynchronize(e->stream));
+                if (own) hipFree(drec);
+            }
+            hipFree(dsch);
+        }
+        if (pending) {
+            hipFree(e->pend_psv); hipFree(e->pend_sh);
+            e->pend_psv = e->pend_sh = nullptr; e->pend_nt = 0;
+        }
+        d.G = G; d.Gv = Gv;
+    }
+    /* receivers and traces */
+    {
+        int *rec = nullptr;
+        float *tr = nullptr;
+        std::vector<int> off(std::max(p->ntrace, 1), 0);
+        for (int i = 0; i < p->ntrace; ++i) off[i] = p->rec_iz[i] * s.ld + p->rec_ix[i];
+        ACHK(dalloc(s, &rec, off.size()));
+        ACHK(dalloc(s, &tr, (size_t)p->nst * 3 * std::max(p->ntrace, 1)));
+        ACHK(hipMemcpyAsync(rec, off.data(), off.size() * 4, hipMemcpyHostToDevice, e->stream));
+        ACHK(hipMemsetAsync(tr, 0, (size_t)p->nst * 3 * std::max(p->ntrace, 1) * 4, e->stream));
+        d.rec = rec; d.traces = tr;
+    }
+    /* level 0 */
+    hipLaunchKernelGGL(init_state_kernel, dim3(64), dim3(256), 0, e->stream, d);
+    if (p->ntrace > 0) hipLaunchKernelGGL(record0_kernel, dim3(1), dim3(256), 0, e->stream, d);
+    {
+        int zero = 0;
+        ACHK(hipMemcpyAsync(e->d_levels + k, &zero, sizeof zero, hipMemcpyHostToDevice, e->stream));
+        ACHK(hipStreamSynchronize(e->stream));
+    }
+    if (hipGetLastError() != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "setup kernel launch failed"); goto fail; }
+#undef ACHK
+    s.used = true;
+    e->used += s.bytes;
+    e->h_levels[k] = 0;
+    e->dirty = true;
+    *slot_out = k;
+    return SEMSKS_OK;
+fail:
+    hipStreamSynchronize(e->stream);
+    for (void *q : s.allocs) hipFree(q);
+    if (share >= 0) e->slots[share].refs--;
+    s = Slot();
+    return st;
+}
+
+semsks_status semsks_retire(semsks_engine *e, int k)
+{
+    if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
+    if (e->slots[k].refs > 0) return semsks_set_error(SEMSKS_EINVAL, "slot %d still shares its material with %d slots", k, e->slots[k].refs);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    free_slot(e, k);
+    e->h_levels[k] = -1;
+    e->dirty = true;
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
+{
+    if (!e || nsteps < 0) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
+    if (e->dirty) { semsks_status st = upload_tables(e); if (st) return st; }
+    bool any = false;
+    for (size_t k = 0; k < e->slots.size(); ++k)
+        if (e->slots[k].used && e->h_levels[k] < e->slots[k].meta.nst - 1) any = true;
+    if (!any || nsteps == 0) { e->last_ms[0] = e->last_ms[1] = 0; return SEMSKS_OK; }
+    HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    double kern_ms = 0.0;
+    e->timed_steps = 0;
+    if (e->opts.use_graph) {
+        const int G = 16;
+        int done = 0;
+        if (nsteps >= G) {
+            if (!e->gexec || e->graph_steps != G || e->graph_mask != e->hpn_mask) {
+                semsks_status st = build_graph(e, G);
+                if (st) return st;
+            }
+            for (; done + G <= nsteps; done += G) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+        }
+        for (; done < nsteps; ++done) { semsks_status st = launch_step(e, nullptr, nullptr); if (st) return st; }
+    } else {
+        /* eager launches with an event pair around every step-kernel launch
+           (the first 1024 steps of the call are timed) */
+        const int ntimed = sync ? std::min(nsteps, 1024) : 0;
+        while ((int)e->tev.size() < 2 * ntimed) {
+            hipEvent_t v;
+            HIPCHK(hipEventCreate(&v));
+            e->tev.push_back(v);
+        }
+        for (int i = 0; i < nsteps; ++i) {
+            semsks_status st = i < ntimed ? launch_step(e, e->tev[2 * i], e->tev[2 * i + 1]) : launch_step(e, nullptr, nullptr);
+            if (st) return st;
+        }
+        if (ntimed) {
+            HIPCHK(hipEventSynchronize(e->tev[2 * ntimed - 1]));
+            for (int i = 0; i < ntimed; ++i) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e->tev[2 * i], e->tev[2 * i + 1]));
+                kern_ms += ms;
+            }
+            e->timed_steps = ntimed;
+        }
+    }
+    HIPCHK(hipEventRecord(e->ev[1], e->stream));
+    for (size_t k = 0; k < e->slots.size(); ++k)
+        if (e->slots[k].used) e->h_levels[k] = std::min(e->h_levels[k] + nsteps, e->slots[k].meta.nst - 1);
+    if (sync) {
+        HIPCHK(hipEventSynchronize(e->ev[1]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+        e->last_ms[0] = ms;
+        e->last_ms[1] = kern_ms;
+    }
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_last_timing(semsks_engine *e, double out[2])
+{
+    out[0] = e->last_ms[0];
+    out[1] = e->last_ms[1];
+    return SEMSKS_OK;
+}
+
+int semsks_last_timed_steps(semsks_engine *e) { return e ? e->timed_steps : 0; }
+
+semsks_status semsks_sync(semsks_engine *e)
+{
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_slot_status(semsks_engine *e, int k, int *level, int *done)
+{
+    if (!e || k < 0 || k >= (int)e->slots.size()) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
+    if (!e->slots[k].used) { *level = -1; *done = 0; return SEMSKS_OK; }
+    *level = e->h_levels[k];
+    *done = e->h_levels[k] >= e->slots[k].meta.nst - 1;
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_get_traces(semsks_engine *e, int k, int it0, int n, float *out)
+{
+    if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
+    const Slot &s = e->slots[k];
+    if (it0 < 0 || n < 0 || it0 + n > s.meta.nst) return semsks_set_error(SEMSKS_EINVAL, "trace range");
+    const size_t w = (size_t)3 * s.meta.ntrace;
+    if (w && n) HIPCHK(hipMemcpyAsync(out, s.d.traces + (size_t)it0 * w, (size_t)n * w * 4, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_get_field(semsks_engine *e, int k, float *out)
+{
+    if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
+    const Slot &s = e->slots[k];
+    const int p = e->h_levels[k] & 1;
+    for (int c = 0; c < 3; ++c)
+        HIPCHK(hipMemcpy2DAsync(out + (size_t)c * s.nx * s.nz, s.nx * 4, s.d.u[p][c], s.ld * 4, s.nx * 4, s.nz,
+                                hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
+{
+    if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
+    const Slot &s = e->slots[k];
+    const size_t nodes = (size_t)s.ld * s.nz;
+    float *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, nodes * 6 * sizeof(float) + 256));
+    HIPCHK(hipMemsetAsync(buf, 0, nodes * 6 * sizeof(float), e->stream));
+    InstDesc d = s.d;
+    for (int c = 0; c < 3; ++c) {
+        d.u[0][c] = buf + c * nodes;
+        d.fout[c] = buf + (3 + c) * nodes;
+        HIPCHK(hipMemcpy2DAsync(d.u[0][c], s.ld * 4, u + (size_t)c * s.nx * s.nz, s.nx * 4, s.nx * 4, s.nz,
+                                hipMemcpyHostToDevice, e->stream));
+    }
+    d.active = 1;
+    InstDesc *dd = nullptr;
+    int4 *tl = nullptr;
+    int *nt = nullptr;
+    std::vector<int4> t;
+    const int tz = tile_rows(e, s.meta.nez);
+    for (int z0 = 0; z0 < s.meta.nez; z0 += tz)
+        for (int x0 = 0; x0 < s.meta.nex; x0 += SEMSKS_XCHUNK) t.push_back(make_int4(0, x0, z0, std::min(tz, s.meta.nez - z0)));
+    int cnt = (int)t.size();
+    HIPCHK(hipMalloc(&dd, sizeof d));
+    HIPCHK(hipMalloc(&tl, t.size() * sizeof(int4)));
+    HIPCHK(hipMalloc(&nt, sizeof(int)));
+    HIPCHK(hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(tl, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(nt, &cnt, sizeof cnt, hipMemcpyHostToDevice, e->stream));
+    const int nb = persistent_blocks(e);
+    switch (s.meta.hpn) {
+    case 3: hipLaunchKernelGGL((step_kernel<3, true>), dim3(nb), dim3(256), 0, e->stream, dd, e->d_levels, tl, nt); break;
+    case 4: hipLaunchKernelGGL((step_kernel<4, true>), dim3(nb), dim3(256), 0, e->stream, dd, e->d_levels, tl, nt); break;
+    case 5: hipLaunchKernelGGL((step_kernel<5, true>), dim3(nb), dim3(256), 0, e->stream, dd, e->d_levels, tl, nt); break;
+    case 6: hipLaunchKernelGGL((step_kernel<6, true>), dim3(nb), dim3(256), 0, e->stream, dd, e->d_levels, tl, nt); break;
+    }
+    HIPCHK(hipGetLastError());
+    for (int c = 0; c < 3; ++c)
+        HIPCHK(hipMemcpy2DAsync(f + (size_t)c * s.nx * s.nz, s.nx * 4, d.fout[c], s.ld * 4, s.nx * 4, s.nz,
+                                hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    hipFree(buf); hipFree(dd); hipFree(tl); hipFree(nt);
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_get_setup(semsks_engine *e, int k, double *mass, float *mdt2, double *ab)
+{
+    if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
+    const Slot &s = e->slots[k];
+    const Slot &o = s.share >= 0 ? e->slots[s.share] : s;
+    const size_t N = (size_t)s.nx * s.nz;
+    double *w64 = nullptr, *dm = nullptr, *dab = nullptr;
+    float *md = nullptr, *ea = nullptr, *eb = nullptr;
+    const int ne = semsks_n_edge(s.nx, s.nz, s.meta.hpn);
+    HIPCHK(hipMalloc(&w64, SEMSKS_MAXNP * sizeof(double)));
+    HIPCHK(hipMalloc(&dm, N * sizeof(double)));
+    HIPCHK(hipMalloc(&dab, N * 3 * sizeof(double)));
+    HIPCHK(hipMalloc(&md, (size_t)s.ld * s.nz * sizeof(float)));
+    HIPCHK(hipMalloc(&ea, ne * 3 * sizeof(float)));
+    HIPCHK(hipMalloc(&eb, ne * 3 * sizeof(float)));
+    HIPCHK(hipMemsetAsync(dab, 0, N * 3 * sizeof(double), e->stream));
+    InstDesc d = o.d, *dd = nullptr;
+    d.mdt2 = md;
+    HIPCHK(hipMalloc(&dd, sizeof d));
+    HIPCHK(hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, e->stream));
+    const double eh = s.meta.eh;
+    hipLaunchKernelGGL(setup_tables_kernel, dim3(1), dim3(64), 0, e->stream, dd, w64, eh);
+    hipLaunchKernelGGL(mass_kernel, dim3(1024), dim3(256), 0, e->stream, d, w64, (double)(eh * eh / 4.0), s.meta.sdt, dm);
+    hipLaunchKernelGGL(absorb_kernel, dim3(64), dim3(256), 0, e->stream, d, w64, eh / 2.0, s.meta.sdt, s.meta.absorb, dm, dab, ea, eb);
+    HIPCHK(hipGetLastError());
+    if (mass) HIPCHK(hipMemcpyAsync(mass, dm, N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    if (ab) HIPCHK(hipMemcpyAsync(ab, dab, N * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    if (mdt2) HIPCHK(hipMemcpy2DAsync(mdt2, s.nx * 4, md, s.ld * 4, s.nx * 4, s.nz, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    hipFree(w64); hipFree(dm); hipFree(dab); hipFree(md); hipFree(ea); hipFree(eb); hipFree(dd);
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_get_boundary_series(semsks_engine *e, int k, int n, float *g, float *gv)
+{
+    if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
+    const Slot &s = e->slots[k];
+    if (n < 0 || n > s.meta.nst) return semsks_set_error(SEMSKS_EINVAL, "n out of range");
+    const int mt = 2 * (s.nx + s.nz), sht = s.nx + s.nz;
+    /* levels at or beyond nt_sem keep the last interpolated vector */
+    for (int i = 0; i < n; ++i) {
+        int src = std::min(i, s.nlev - 1);
+        if (s.nt_sem > 0) src = std::min(src, s.nt_sem - 1);
+        HIPCHK(hipMemcpyAsync(g + (size_t)i * mt, s.d.G + (size_t)src * mt, mt * 4, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipMemcpyAsync(gv + (size_t)i * sht, s.d.Gv + (size_t)src * sht, sht * 4, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_planewave(semsks_engine *e, int nex, int nez, int hpn, int neh, float eh, int nt, float dp,
+                               double p, double beta, double f0, double psi, float *psv, float *sh)
+{
+    if (!e || hpn < 3 || hpn > 6 || nex < 1 || nez < 1 || nt < 1 || !(p * beta >= 0 && p * beta < 1))
+        return semsks_set_error(SEMSKS_EINVAL, "bad plane-wave arguments");
+    const int nx = nex * hpn + 1, nz = nez * hpn + 1, mt = 2 * (nx + nz), sht = nx + nz;
+    float *dpsv = nullptr, *dsh = nullptr;
+    HIPCHK(hipMalloc(&dpsv, (size_t)nt * mt * 4));
+    HIPCHK(hipMalloc(&dsh, (size_t)nt * sht * 4));
+    hipLaunchKernelGGL(planewave_kernel, dim3(2048), dim3(256), 0, e->stream, nex, nez, hpn, neh, (double)eh, nt,
+                       (double)dp, p, beta, f0, psi, dpsv, dsh);
+    HIPCHK(hipGetLastError());
+    if (psv || sh) {
+        if (psv) HIPCHK(hipMemcpyAsync(psv, dpsv, (size_t)nt * mt * 4, hipMemcpyDeviceToHost, e->stream));
+        if (sh) HIPCHK(hipMemcpyAsync(sh, dsh, (size_t)nt * sht * 4, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        hipFree(dpsv); hipFree(dsh);
+        return SEMSKS_OK;
+    }
+    if (e->pend_psv) hipFree(e->pend_psv);
+    if (e->pend_sh) hipFree(e->pend_sh);
+    e->pend_psv = dpsv; e->pend_sh = dsh; e->pend_nt = nt; e->pend_nx = nx; e->pend_nz = nz;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_run_problem(const semsks_problem *p, float *traces)
+{
+    semsks_engine_opts o{};
+    o.max_slots = 1;
+    o.use_graph = 1;
+    semsks_engine *e = nullptr;
+    semsks_status st = semsks_engine_create(&o, &e);
+    if (st) return st;
+    int slot = -1;
+    st = semsks_admit(e, p, -1, &slot);
+    if (!st) st = semsks_advance(e, p->nst - 1, 1);
+    if (!st && traces) st = semsks_get_traces(e, slot, 0, p->nst, traces);
+    semsks_engine_destroy(e);
+    return st;
+}
+
+}  /* extern "C" */

@@ -1,0 +1,61 @@
+/*
+ * semsks_kernels.h -- device data layout shared by the kernels and the engine.
+ *
+ * Layout in HBM, per instance (slot):
+ *   u[2][3]   displacement, ping-pong pair of SoA planes (x, y, z); plane
+ *             row stride ld floats (ld >= nx+3, multiple of 64) so that an
+ *             element's first hpn columns are one aligned vector load
+ *   del[3]    delta = u(t) - u(t-dt), in place
+ *   cmp[3]    Kahan compensation of u, in place (absent without Kahan)
+ *   mat_id    16-bit material record id per node (same row stride)
+ *   emat      16-bit record id per element when all its nodes share one
+ *             record, 0xFFFF otherwise
+ *   mat       records {rho, 15 constants}, 16 floats each
+ *   mdt2      dt^2/M per node (FP32, from the FP64 lumped mass)
+ *   edge_a/b  (M-A)/(M+A) and dt^2/(M+A) per absorbing-edge node, 3 comps
+ *   G, Gv     interpolated, guarded boundary vectors per time level
+ *   traces    [nst][3][ntrace] receiver samples (u, -w, v)
+ */
+#ifndef SEMSKS_KERNELS_H
+#define SEMSKS_KERNELS_H
+
+#include <stdint.h>
+
+#define SEMSKS_MAXNP 7
+#define SEMSKS_XCHUNK 63   /* owned element columns per 64-lane wave tile */
+
+struct InstDesc {
+    int nex, nez, nx, nz, ld;
+    int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active;
+    float *u[2][3];
+    float *del[3];
+    float *cmp[3];
+    float *fout[3];                 /* force-only mode output, else null */
+    const uint16_t *mat_id;
+    const uint16_t *emat;
+    const float *mat;
+    const float *mdt2;
+    const float *edge_a, *edge_b;   /* [n_edge][3] */
+    const float *G, *Gv;            /* [nt_sem][2(nx+nz)], [nt_sem][nx+nz] */
+    const int *rec;                 /* node offsets iz*ld+ix */
+    float *traces;
+    /* element operators (FP32 from FP64):
+       Dg[a][t] = l'_t(xi_a) * 2/eh      (gradient, dlagP_JeI)
+       Dw[a][k] = l'_k(xi_a) * w_a       (divergence, dlagP_Gllw)
+       cw[k]    = Je * Je^-1 * w_k = (eh/2) w_k                       */
+    float Dg[SEMSKS_MAXNP][SEMSKS_MAXNP];
+    float Dw[SEMSKS_MAXNP][SEMSKS_MAXNP];
+    float cw[SEMSKS_MAXNP];
+    int pad[5];
+};
+
+/* edge-node numbering: bottom row iz=0 (nx) | left column iz>=1 (nz-1) |
+   right column iz>=1 (nz-1) | top row iz=nz-1, ix=1..3*hpn (3*hpn) */
+#ifdef __HIPCC__
+#define SEMSKS_HD __host__ __device__
+#else
+#define SEMSKS_HD
+#endif
+static inline SEMSKS_HD int semsks_n_edge(int nx, int nz, int hpn) { return nx + 2 * (nz - 1) + 3 * hpn; }
+
+#endif

@@ -1,0 +1,165 @@
+"""HIP path vs the CPU oracle on the same inputs (the parity tests proper).
+
+Bars:
+  * setup products that are integer or FP64 work on both sides (material per
+    node, receiver indices, FP64 lumped mass, Lysmer coefficients, the FP32
+    interpolated boundary series): bit-exact;
+  * FP32 element force K u: max |f - f64| <= 2e-6 * max |f64|;
+  * seismograms (FP32 time loop) vs the FP64 oracle: relative L2 <= 1e-4 and
+    peak misfit <= 1e-4 of the peak, per component with signal.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle
+import semsks
+from semsks import inputs
+
+pytestmark = pytest.mark.gpu
+
+REL_L2_TOL = 1e-4
+PEAK_TOL = 1e-4
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def peak_misfit(a, b):
+    pb = np.abs(b).max()
+    return float(np.abs(a - b).max() / pb) if pb > 0 else float(np.abs(a).max())
+
+
+def make_case(tmp_path, **kw):
+    c = inputs.small_case(**kw)
+    d = os.fspath(tmp_path)
+    c.write(d)
+    return d
+
+
+@pytest.fixture
+def small(tmp_path, monkeypatch):
+    d = make_case(tmp_path)
+    monkeypatch.chdir(d)
+    return d
+
+
+def test_setup_bit_exact(small):
+    print("case: small 24x12, setup parity", flush=True)
+    m = semsks.Model(["par=run.par"])
+    o64 = pyoracle.Oracle(["par=run.par"], "f64")
+    with semsks.Engine() as e:
+        s = e.admit(m.problem)
+        mass, mdt2, ab = e.setup(s)
+    om, omd, oab = o64.mass()
+    assert np.array_equal(mass, om)
+    assert np.array_equal(mdt2, omd.astype(np.float32))
+    assert np.array_equal(ab, oab)
+
+
+def test_boundary_series_bit_exact(small):
+    print("case: small, boundary series", flush=True)
+    m = semsks.Model(["par=run.par"])
+    o32 = pyoracle.Oracle(["par=run.par"], "f32")
+    n = 120
+    with semsks.Engine() as e:
+        s = e.admit(m.problem)
+        g, gv = e.boundary_series(s, n)
+    og, ogv = o32.boundary_series(n)
+    assert np.array_equal(g.view(np.uint32), og.view(np.uint32))
+    assert np.array_equal(gv.view(np.uint32), ogv.view(np.uint32))
+    assert np.abs(g).max() > 0.1
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_element_force(small, seed):
+    print("case: small, K u seed", seed, flush=True)
+    m = semsks.Model(["par=run.par"])
+    o64 = pyoracle.Oracle(["par=run.par"], "f64")
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((3, m.nz, m.nx)).astype(np.float32)
+    with semsks.Engine() as e:
+        s = e.admit(m.problem)
+        f = e.stiff(s, u)
+    f64 = o64.stiff(u.astype(np.float64))
+    err = np.abs(f - f64).max() / np.abs(f64).max()
+    assert err < 2e-6, err
+
+
+def run_gpu(problem, **eng):
+    with semsks.Engine(**eng) as e:
+        s = e.admit(problem)
+        e.advance(problem.nst - 1)
+        return e.traces(s)
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2])
+def test_seismograms_vs_fp64(tmp_path, monkeypatch, solver):
+    print("case: small, seismograms, solver", solver, flush=True)
+    d = make_case(tmp_path, solver=solver, sh_zero=(solver != 0), psi_deg=30.0 if solver == 0 else 0.0)
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    tr = run_gpu(m.problem)
+    ref = pyoracle.Oracle(["par=run.par"], "f64").run()
+    assert tr.shape == ref.shape
+    for k in range(3):
+        if np.abs(ref[:, k]).max() == 0:
+            assert np.abs(tr[:, k]).max() == 0
+            continue
+        assert rel_l2(tr[:, k], ref[:, k]) <= REL_L2_TOL, (k, rel_l2(tr[:, k], ref[:, k]))
+        assert peak_misfit(tr[:, k], ref[:, k]) <= PEAK_TOL
+
+
+def test_cli_dropin(small):
+    print("case: small, CLI", flush=True)
+    os.makedirs("gpu", exist_ok=True)
+    os.makedirs("cpu", exist_ok=True)
+    r = subprocess.run([semsks.CLI_PATH, "par=run.par", "outdir=gpu"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert pyoracle.main(["par=run.par"], "cpu", kind="f64") == 0
+    for comp in ("u", "w", "v"):
+        a = np.fromfile("gpu/seis." + comp, np.float32)
+        b = np.fromfile("cpu/seis." + comp, np.float32)
+        assert a.size == b.size == 600 * 8
+        if np.abs(b).max() > 0:
+            assert rel_l2(a, b) <= REL_L2_TOL
+
+
+def test_batch_members_match_solo(tmp_path, monkeypatch):
+    """Instances stepped together in one launch give the same traces as alone."""
+    print("case: small, batch of 3 vs solo", flush=True)
+    probs = []
+    for k, (p, psi) in enumerate([(0.04, 0.0), (0.06, 30.0), (0.07, 75.0)]):
+        d = make_case(tmp_path / ("c%d" % k), p=p, psi_deg=psi, sh_zero=False)
+        monkeypatch.chdir(d)
+        probs.append(semsks.Model(["par=run.par"]))
+    solo = [run_gpu(m.problem) for m in probs]
+    with semsks.Engine(max_slots=3) as e:
+        slots = [e.admit(probs[0].problem)]
+        slots += [e.admit(m.problem, share=slots[0]) for m in probs[1:]]
+        e.advance(probs[0].problem.nst - 1)
+        batch = [e.traces(s) for s in slots]
+    for a, b in zip(batch, solo):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_dipping_interface_and_tilted_axis(tmp_path, monkeypatch):
+    """Non-flat interfaces (per-node material path) and a tilted-axis layer
+    (15 constants) against the FP64 oracle."""
+    print("case: small, dipping interface + tilted axis", flush=True)
+    from semsks import aniso
+    C = aniso.hexagonal_tilted(vp=7.8, vs=4.4, rho=3.3, strength=0.04, dip_deg=30.0, azimuth_deg=60.0)
+    xmax = 24 * 2.0 + 2.0
+    interfaces = [([0.0, xmax], [14.0, 18.0]), None, None]
+    d = make_case(tmp_path, interfaces=interfaces, aniso=[C, None, None], psi_deg=20.0, sh_zero=False)
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    tr = run_gpu(m.problem)
+    o = pyoracle.Oracle(["par=run.par"], "f64")
+    ref = o.run()
+    for k in range(3):
+        assert rel_l2(tr[:, k], ref[:, k]) <= REL_L2_TOL, (k, rel_l2(tr[:, k], ref[:, k]))
