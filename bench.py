@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""GLL point-steps/s of the SEM time loop on one MI355X.
+
+Default workload (the configuration the metric is quoted on): the uniform
+batch -- 64 incident plane S waves (16 back-azimuths x 4 ray parameters) on
+the 2000x500-element mesh (16,010,001 GLL points each), stepped together by
+one persistent kernel per time step captured in a hipGraph.  The same run
+also times the plain single-instance path on that mesh (the production
+anisotropic model) and the faithful CPU port on the host.
+
+value       = sum_instances nglob * steps / wall seconds of K timed steps
+              (inputs resident in HBM, barrier + device sync both sides)
+roofline    = algorithmic bytes per launch of the fused element-force/update
+              kernel / its average launch time (HIP events on the engine's
+              stream), against MI355X's 8 TB/s HBM
+cpu_baseline= the faithful FP32 port (oracle/sem_oracle.c, -O3, OpenMP node
+              loops as the reference) on a bounded sample of the same mesh
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3|c2|c1b|c4]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "s-sks_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GLL point-steps/s batched FP32 vs plain; p50/p99 latency per synthetic"
+
+
+def alg_bytes(n_c: int, share: int) -> float:
+    """Compulsory bytes per instance-GLL-point-step of the reference algorithm:
+    read u(t), u(t-dt), write u(t+dt) (3 comps, FP32) + n_C constants and
+    the inverse mass per node, shared by `share` instances."""
+    return 36.0 + 4.0 * (n_c + 1) / share
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0):
+    """Admits one instance per (built, p, psi) entry (boundary records made by
+    the device plane-wave generator), runs warmup + timed steps in hipGraph
+    mode.  Returns throughput (pt-steps/s), ms/step and the live engine."""
+    import semsks
+    nt = warmup + steps + extra_steps + 8
+    eng = semsks.Engine(max_slots=len(waves), tile_z=tile_z, use_graph=1)
+    slots = []
+    t0 = time.time()
+    for k, (b, p, psi) in enumerate(waves):
+        prob = b.problem(nt)
+        prob.nst = nt + 1
+        eng.planewave(*b.planewave_args(nt, p=p, psi=psi), to_host=False)
+        share = slots[0] if (share_material and slots) else -1
+        slots.append(eng.admit(prob, share=share))
+    log("  admitted %d instances in %.1fs" % (len(waves), time.time() - t0))
+    npts = sum(b.nglob() for b, _, _ in waves)
+    eng.advance(warmup, sync=True)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.advance(steps, sync=True)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    return dict(throughput=npts * steps / wall, ms_per_step=1e3 * wall / steps, npts=npts, engine=eng, slots=slots)
+
+
+def kernel_time(eng, nsteps):
+    """Average device time of the fused step kernel over nsteps eager steps."""
+    import semsks
+    semsks._check(semsks.lib().semsks_advance(eng._h, nsteps, 2))
+    total, kern, n = eng.timing()
+    return kern / max(n, 1), total / nsteps
+
+
+def cpu_baseline(case_builder, steps: int, threads: int | None = None):
+    """Faithful FP32 port (-O3) on the host: one instance of the mesh."""
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from semsks import inputs
+    if threads:
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    c = case_builder()
+    with tempfile.TemporaryDirectory() as d:
+        c.nst = steps + 1
+        c.write(d, nt=steps + 8)
+        t0 = time.time()
+        o = pyoracle.Oracle(["par=run.par"], "f32", cwd=d)
+        setup = time.time() - t0
+        t0 = time.perf_counter()
+        o.run(steps + 1)
+        wall = time.perf_counter() - t0
+        nglob = o.nx * o.nz
+        o.close()
+    return dict(value=nglob * steps / wall, steps=steps, wall=wall, setup=setup, nglob=nglob)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--tile-z", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--kernel-steps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import semsks
+    from semsks import configs
+
+    out = {}
+    if args.config == "c3":
+        # plain: one instance of the production anisotropic model (C2)
+        log("plain: C2 mesh, 1 instance")
+        b2 = configs.Built(configs.c2_case(nst=10))
+        r2 = run_engine([(b2, 0.05, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z, extra_steps=args.kernel_steps)
+        k2, _ = kernel_time(r2["engine"], args.kernel_steps)
+        r2["engine"].close()
+        log("  plain %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms" % (r2["throughput"], r2["ms_per_step"], k2))
+        # batched: 64 waves on the C2 mesh with the isotropic model, one material
+        log("batched: C3, %d instances" % args.batch)
+        b3 = configs.Built(configs.c3_case(nst=10))
+        waves = [(b3, p, baz) for p, baz in configs.c3_waves()][: args.batch]
+        r3 = run_engine(waves, args.steps, args.warmup, True, tile_z=args.tile_z, extra_steps=args.kernel_steps)
+        k3, _ = kernel_time(r3["engine"], args.kernel_steps)
+        r3["engine"].close()
+        log("  batched %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms" % (r3["throughput"], r3["ms_per_step"], k3))
+        B = alg_bytes(9, len(waves))
+        achieved = B * r3["npts"] / (k3 * 1e-3) / 1e9
+        value, ms, npts, steps = r3["throughput"], r3["ms_per_step"], r3["npts"], args.steps
+        cfg = {"workload": "uniform batch: %d plane S waves (16 baz x 4 p) on a 2000x500-element NGLL=5 mesh, "
+                           "16,010,001 GLL points each, shared isotropic 3-layer model" % len(waves),
+               "instances": len(waves), "nglob_per_instance": int(b3.nglob()), "mesh": "2000x500 elements",
+               "plain_workload": "1 instance, 2000x500 mesh, tilted-axis hexagonal 4% upper mantle (15 constants)",
+               "plain_value": r2["throughput"], "plain_ms_per_step": r2["ms_per_step"],
+               "batched_over_plain": r3["throughput"] / r2["throughput"],
+               "kernel_ms_batched": k3, "kernel_ms_plain": k2,
+               "bytes_per_point_step_alg": B, "kahan": True}
+        cpu_case = lambda: configs.c2_case(nst=10)
+    elif args.config == "c2":
+        b2 = configs.Built(configs.c2_case(nst=10))
+        r2 = run_engine([(b2, 0.05, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z, extra_steps=args.kernel_steps)
+        k2, _ = kernel_time(r2["engine"], args.kernel_steps)
+        r2["engine"].close()
+        B = alg_bytes(15, 1)
+        achieved = B * r2["npts"] / (k2 * 1e-3) / 1e9
+        value, ms, npts, steps = r2["throughput"], r2["ms_per_step"], r2["npts"], args.steps
+        cfg = {"workload": "1 instance, 2000x500 mesh, tilted-axis hexagonal 4% upper mantle", "instances": 1,
+               "kernel_ms": k2, "bytes_per_point_step_alg": B}
+        cpu_case = lambda: configs.c2_case(nst=10)
+    elif args.config == "c1b":
+        # the literal reading: 64 waves on the C1 mesh (321,201 points each)
+        b1 = configs.Built(configs.c1_case(nst=10, solver=0, sh_zero=False))
+        waves = [(b1, p, baz) for p, baz in configs.c3_waves()][: args.batch]
+        r1 = run_engine([(b1, 0.06, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z, extra_steps=args.kernel_steps)
+        k1, _ = kernel_time(r1["engine"], args.kernel_steps)
+        r1["engine"].close()
+        rb = run_engine(waves, args.steps, args.warmup, True, tile_z=args.tile_z, extra_steps=args.kernel_steps)
+        kb, _ = kernel_time(rb["engine"], args.kernel_steps)
+        rb["engine"].close()
+        B = alg_bytes(9, len(waves))
+        achieved = B * rb["npts"] / (kb * 1e-3) / 1e9
+        value, ms, npts, steps = rb["throughput"], rb["ms_per_step"], rb["npts"], args.steps
+        cfg = {"workload": "%d plane waves on the 200x100 mesh (321,201 points each)" % len(waves),
+               "plain_value": r1["throughput"], "batched_over_plain": rb["throughput"] / r1["throughput"],
+               "kernel_ms_batched": kb, "kernel_ms_plain": k1, "bytes_per_point_step_alg": B}
+        cpu_case = lambda: configs.c1_case(nst=10)
+    else:
+        raise SystemExit("unknown config " + args.config)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        log("cpu baseline: %d threads, %d steps" % (threads, args.cpu_steps))
+        cb = cpu_baseline(cpu_case, args.cpu_steps, threads)
+        cpu = {"value": cb["value"], "unit": "GLL point-steps/s", "cores": threads, "kind": "port",
+               "sample": "faithful FP32 port of sem2d_psvsh (oracle/sem_oracle.c, -O3, OpenMP on the node "
+                         "loops as the reference), 1 instance of the %d-point mesh, %d steps (%.1f s)"
+                         % (cb["nglob"], cb["steps"], cb["wall"])}
+        log("  cpu %.3e pt-steps/s" % cb["value"])
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "GLL point-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (closed-form plane S waves, seeded models)",
+        "config": cfg,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["config"]["vs_cpu"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

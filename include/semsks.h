@@ -152,7 +152,9 @@ size_t semsks_footprint(const semsks_problem *p, int shared_material);
 /* Advances every active instance by up to nsteps time steps (the loop body
    of sem2d_psvsh.c:418-512: GRT_Interface4/3, Stiff_Matrix,
    Time_diff_hybrid + Absorbs, the level rotation and kirrecord_zhl).
-   Instances stop by themselves at their nst. Asynchronous unless sync. */
+   Instances stop by themselves at their nst.  sync = 0: asynchronous;
+   1: returns when the steps are done; 2: as 1, but launched eagerly (no
+   graph) with every step-kernel launch timed (semsks_last_timing). */
 semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync);
 
 /* Current time level of a slot (-1 if empty) and whether it finished. */

@@ -292,9 +292,9 @@ __device__ __forceinline__ void kahan_add(float &u, float &c, float x)
    reference then copies the changed u(t) into u(t-dt), so delta moves with u. */
 __global__ void inject_kernel(const InstDesc *insts, const int *levels)
 {
-    const InstDesc &D = insts[blockIdx.x];
+    const InstDesc &D = insts[blockIdx.y];
     if (!D.active) return;
-    const int it = levels[blockIdx.x];
+    const int it = levels[blockIdx.y];
     if (it >= D.nst - 1) return;
     const int lev = it + 1;
     if (lev >= D.nt_sem) return;
@@ -304,7 +304,7 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
     const bool sh = D.solver == SEMSKS_SOLVER_PSVSH, mp = D.solver == SEMSKS_SOLVER_PSV_MP;
     float *U[3] = {D.u[p][0], D.u[p][1], D.u[p][2]};
     const int nb = nx - nbl, nl = nz - nbl - 1;
-    for (int i = threadIdx.x; i < nb + nl; i += blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb + nl; i += gridDim.x * blockDim.x) {
         long q;
         float d[3];
         if (i < nb) {
@@ -380,9 +380,58 @@ __global__ void record0_kernel(InstDesc D)
 /* ========================================================================= */
 /* the element force + update kernel                                         */
 /* ========================================================================= */
-struct MatScalar {
-    float C[15];
+/* Compile-time element operators.  For square elements the 2-D stiffness
+   is independent of the element size (gradient ~ 2/eh, quadrature weight ~
+   (eh/2)^2), so the kernel uses the reference-element operators
+     D[a][t]  = l'_t(x_a)                  (dLag_N_i, lagrange.c:42-67)
+     Dw[a][k] = l'_k(x_a) w_a              (dlagP_Gllw, sem2d_psvsh.c:299-300)
+   evaluated in double at compile time from the GLL nodes and rounded to
+   FP32.  Only the mass (dt^2/M) carries eh. */
+template <int N> struct GllNodes;
+template <> struct GllNodes<3> { static constexpr double x[4] = {-1.0, -0.44721359549995793928, 0.44721359549995793928, 1.0}; };
+template <> struct GllNodes<4> { static constexpr double x[5] = {-1.0, -0.65465367070797714380, 0.0, 0.65465367070797714380, 1.0}; };
+template <> struct GllNodes<5> { static constexpr double x[6] = {-1.0, -0.76505532392946469285, -0.28523151648064509632, 0.28523151648064509632, 0.76505532392946469285, 1.0}; };
+template <> struct GllNodes<6> { static constexpr double x[7] = {-1.0, -0.83022389627856692987, -0.46884879347071421381, 0.0, 0.46884879347071421381, 0.83022389627856692987, 1.0}; };
+
+template <int N>
+struct ElemOps {
+    float D[N + 1][N + 1];
+    float Dw[N + 1][N + 1];
+    float w[N + 1];
+    constexpr ElemOps() : D(), Dw(), w()
+    {
+        const double *x = GllNodes<N>::x;
+        double wd[N + 1] = {};
+        for (int i = 0; i <= N; ++i) {
+            if (i == 0 || i == N) { wd[i] = 2.0 / (N * (N + 1)); continue; }
+            double p2 = 1.0, p1 = x[i], pn = 0.0;
+            for (int k = 2; k <= N; ++k) { pn = ((2 * k - 1) * x[i] * p1 - (k - 1) * p2) / k; p2 = p1; p1 = pn; }
+            wd[i] = 2.0 / (N * (N + 1) * pn * pn);
+        }
+        for (int a = 0; a <= N; ++a) {
+            w[a] = (float)wd[a];
+            for (int t = 0; t <= N; ++t) {
+                double down = 1.0, v = 0.0;
+                for (int i = 0; i <= N; ++i) if (i != t) down *= (x[t] - x[i]);
+                for (int i = 0; i <= N; ++i) {
+                    if (i == t) continue;
+                    double L = 1.0;
+                    for (int m = 0; m <= N; ++m) if (m != t && m != i) L *= (x[a] - x[m]);
+                    v += L;
+                }
+                D[a][t] = (float)(v / down);
+                Dw[a][t] = (float)(v / down * wd[a]);
+            }
+        }
+    }
 };
+
+template <int N>
+__device__ __forceinline__ constexpr float opD(int a, int t) { constexpr ElemOps<N> o{}; return o.D[a][t]; }
+template <int N>
+__device__ __forceinline__ constexpr float opDw(int a, int t) { constexpr ElemOps<N> o{}; return o.Dw[a][t]; }
+template <int N>
+__device__ __forceinline__ constexpr float opW(int a) { constexpr ElemOps<N> o{}; return o.w[a]; }
 
 template <int N>
 __device__ __forceinline__ void load_cols(const float *__restrict__ row, int ex, bool valid, bool direct_last,
@@ -417,11 +466,11 @@ __device__ __forceinline__ void load_ids(const uint16_t *__restrict__ row, int e
 }
 
 /* stress at one node from the 5x5 block C over {xx, zz, yz, xz, xy} and the
-   2-D strains (uy,y = ux,y = uz,y = 0, Stress_ij sem2d_psvsh.c:1345-1379) */
+   2-D strains (uy,y = ux,y = uz,y = 0, Stress_ij sem2d_psvsh.c:1345-1379);
+   C: 0:00 1:02 2:03 3:04 4:05 5:22 6:23 7:24 8:25 9:33 10:34 11:35 12:44 13:45 14:55 */
 __device__ __forceinline__ void stress5(const float *C, float e0, float e2, float e3, float e4, float e5,
                                         float &s0, float &s2, float &s3, float &s4, float &s5)
 {
-    /* C: 0:00 1:02 2:03 3:04 4:05 5:22 6:23 7:24 8:25 9:33 10:34 11:35 12:44 13:45 14:55 */
     s0 = C[0] * e0 + C[1] * e2 + C[2] * e3 + C[3] * e4 + C[4] * e5;
     s2 = C[1] * e0 + C[5] * e2 + C[6] * e3 + C[7] * e4 + C[8] * e5;
     s3 = C[2] * e0 + C[6] * e2 + C[9] * e3 + C[10] * e4 + C[11] * e5;
@@ -429,19 +478,15 @@ __device__ __forceinline__ void stress5(const float *C, float e0, float e2, floa
     s5 = C[4] * e0 + C[8] * e2 + C[11] * e3 + C[13] * e4 + C[14] * e5;
 }
 
+/* K_e u_e of one element (Stiff_Matrix's dUj_di_2D -> Stress_ij -> F_ik ->
+   Grid_Stiff), accumulated onto f (whose bottom row may already hold the
+   carried top row of the element below) */
 template <int N, bool UNIFORM>
 __device__ __forceinline__ void element_force(const float (&u)[N + 1][N + 1][3], float (&f)[N + 1][N + 1][3],
-                                              const float (&Dg)[N + 1][N + 1], const float (&Dw)[N + 1][N + 1],
-                                              const float (&cw)[N + 1], const float *Cu, const float *__restrict__ mat,
+                                              const float *Cu, const float *__restrict__ mat,
                                               const int (&ids)[N + 1][N + 1])
 {
     constexpr int NP = N + 1;
-#pragma unroll
-    for (int a = 0; a < NP; ++a)
-#pragma unroll
-        for (int b = 0; b < NP; ++b)
-#pragma unroll
-            for (int i = 0; i < 3; ++i) f[a][b][i] = 0.f;
 #pragma unroll
     for (int b = 0; b < NP; ++b) {
 #pragma unroll
@@ -449,12 +494,13 @@ __device__ __forceinline__ void element_force(const float (&u)[N + 1][N + 1][3],
             float gx0 = 0.f, gx1 = 0.f, gx2 = 0.f, gz0 = 0.f, gz1 = 0.f, gz2 = 0.f;
 #pragma unroll
             for (int t = 0; t < NP; ++t) {
-                gx0 += Dg[a][t] * u[t][b][0];
-                gx1 += Dg[a][t] * u[t][b][1];
-                gx2 += Dg[a][t] * u[t][b][2];
-                gz0 += Dg[b][t] * u[a][t][0];
-                gz1 += Dg[b][t] * u[a][t][1];
-                gz2 += Dg[b][t] * u[a][t][2];
+                const float dx = opD<N>(a, t), dz = opD<N>(b, t);
+                gx0 += dx * u[t][b][0];
+                gx1 += dx * u[t][b][1];
+                gx2 += dx * u[t][b][2];
+                gz0 += dz * u[a][t][0];
+                gz1 += dz * u[a][t][1];
+                gz2 += dz * u[a][t][2];
             }
             float C[15];
             if constexpr (UNIFORM) {
@@ -469,17 +515,18 @@ __device__ __forceinline__ void element_force(const float (&u)[N + 1][N + 1][3],
             }
             float s0, s2, s3, s4, s5;
             stress5(C, gx0, gz2, gz1, gx2 + gz0, gx1, s0, s2, s3, s4, s5);
-            const float cb = cw[b], ca = cw[a];
-            const float p0 = cb * s0, p1 = cb * s5, p2 = cb * s4;   /* row traction   (T_i0) */
-            const float q0 = ca * s4, q1 = ca * s3, q2 = ca * s2;   /* column traction (T_i2) */
+            const float wb = opW<N>(b), wa = opW<N>(a);
+            const float p0 = wb * s0, p1 = wb * s5, p2 = wb * s4;   /* row traction    T_i0 */
+            const float q0 = wa * s4, q1 = wa * s3, q2 = wa * s2;   /* column traction T_i2 */
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
-                f[k][b][0] += Dw[a][k] * p0;
-                f[k][b][1] += Dw[a][k] * p1;
-                f[k][b][2] += Dw[a][k] * p2;
-                f[a][k][0] += Dw[b][k] * q0;
-                f[a][k][1] += Dw[b][k] * q1;
-                f[a][k][2] += Dw[b][k] * q2;
+                const float ax = opDw<N>(a, k), az = opDw<N>(b, k);
+                f[k][b][0] += ax * p0;
+                f[k][b][1] += ax * p1;
+                f[k][b][2] += ax * p2;
+                f[a][k][0] += az * q0;
+                f[a][k][1] += az * q1;
+                f[a][k][2] += az * q2;
             }
         }
     }
@@ -494,78 +541,95 @@ __device__ __forceinline__ int edge_index(int ix, int iz, int nx, int nz, int n)
     return -1;
 }
 
-/* update one owned node from its assembled force f and its u(t) */
-template <bool FORCE_ONLY>
-__device__ __forceinline__ void update_node(const InstDesc &D, int p, long q, int ix, int iz, const float (&f)[3],
-                                            const float (&uc)[3])
+/* update of the owned nodes (columns a0..a1-1) of one element row.  All
+   loads are issued before any store: the state planes are distinct arrays,
+   but the compiler cannot prove it, so interleaving would serialise the
+   memory round trips. */
+template <int N, bool FORCE_ONLY>
+__device__ __forceinline__ void update_row(const InstDesc &D, int p, long q0, int ix0, int iz, int a1,
+                                           const float (&f)[N + 1][N + 1][3], const float (&u)[N + 1][N + 1][3],
+                                           int b)
 {
-    if constexpr (FORCE_ONLY) {
-        D.fout[0][q] = f[0]; D.fout[1][q] = f[1]; D.fout[2][q] = f[2];
-        return;
-    } else {
-        const int e = edge_index(ix, iz, D.nx, D.nz, D.hpn);
-        const float m = D.mdt2[q];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float d = D.del[k][q];
-            if (e < 0) d = d - m * f[k];
-            else d = D.edge_a[e * 3 + k] * d - D.edge_b[e * 3 + k] * f[k];
-            float un = uc[k];
-            if (D.kahan) {
-                float c = D.cmp[k][q];
-                kahan_add(un, c, d);
-                D.cmp[k][q] = c;
-            } else {
-                un = un + d;
-            }
-            D.del[k][q] = d;
-            D.u[p ^ 1][k][q] = un;
-        }
-    }
-}
-
-/* four consecutive interior-row nodes at once (N == 4 vector path) */
-template <bool FORCE_ONLY>
-__device__ __forceinline__ void update_row4(const InstDesc &D, int p, long q0, int ix0, int iz,
-                                            const float (&f)[5][5][3], const float (&u)[5][5][3], int b)
-{
+    constexpr int NP = N + 1;
     if constexpr (FORCE_ONLY) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            *reinterpret_cast<float4 *>(D.fout[k] + q0) = make_float4(f[0][b][k], f[1][b][k], f[2][b][k], f[3][b][k]);
+        for (int a = 0; a < NP; ++a)
+            if (a < a1)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) D.fout[k][q0 + a] = f[a][b][k];
         return;
     } else {
-        const bool edge = (iz == 0) || (ix0 == 0) || (iz == D.nz - 1 && ix0 <= 3 * D.hpn);
-        if (edge) {
+        const bool kahan = D.kahan != 0;
+        const bool edge_row = (iz == 0) || (iz == D.nz - 1) || (ix0 == 0) || (a1 == NP);
+        if constexpr (N == 4) {
+            if (!edge_row) {
+                const float4 m = *reinterpret_cast<const float4 *>(D.mdt2 + q0);
+                float4 d[3], c[3];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                float fa[3] = {f[a][b][0], f[a][b][1], f[a][b][2]};
-                float ua[3] = {u[a][b][0], u[a][b][1], u[a][b][2]};
-                update_node<false>(D, p, q0 + a, ix0 + a, iz, fa, ua);
+                for (int k = 0; k < 3; ++k) d[k] = *reinterpret_cast<const float4 *>(D.del[k] + q0);
+                if (kahan) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) c[k] = *reinterpret_cast<const float4 *>(D.cmp[k] + q0);
+                }
+                float4 un[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    d[k].x = d[k].x - m.x * f[0][b][k];
+                    d[k].y = d[k].y - m.y * f[1][b][k];
+                    d[k].z = d[k].z - m.z * f[2][b][k];
+                    d[k].w = d[k].w - m.w * f[3][b][k];
+                    un[k] = make_float4(u[0][b][k], u[1][b][k], u[2][b][k], u[3][b][k]);
+                    if (kahan) {
+                        kahan_add(un[k].x, c[k].x, d[k].x);
+                        kahan_add(un[k].y, c[k].y, d[k].y);
+                        kahan_add(un[k].z, c[k].z, d[k].z);
+                        kahan_add(un[k].w, c[k].w, d[k].w);
+                    } else {
+                        un[k].x += d[k].x; un[k].y += d[k].y; un[k].z += d[k].z; un[k].w += d[k].w;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    *reinterpret_cast<float4 *>(D.del[k] + q0) = d[k];
+                    *reinterpret_cast<float4 *>(D.u[p ^ 1][k] + q0) = un[k];
+                    if (kahan) *reinterpret_cast<float4 *>(D.cmp[k] + q0) = c[k];
+                }
+                return;
             }
-            return;
         }
-        const float4 m = *reinterpret_cast<const float4 *>(D.mdt2 + q0);
+        /* general path (edge rows/columns, the domain's last column, N != 4):
+           one node at a time, its loads still ahead of its stores */
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float4 d = *reinterpret_cast<const float4 *>(D.del[k] + q0);
-            d.x = d.x - m.x * f[0][b][k];
-            d.y = d.y - m.y * f[1][b][k];
-            d.z = d.z - m.z * f[2][b][k];
-            d.w = d.w - m.w * f[3][b][k];
-            float4 un = make_float4(u[0][b][k], u[1][b][k], u[2][b][k], u[3][b][k]);
-            if (D.kahan) {
-                float4 c = *reinterpret_cast<const float4 *>(D.cmp[k] + q0);
-                kahan_add(un.x, c.x, d.x);
-                kahan_add(un.y, c.y, d.y);
-                kahan_add(un.z, c.z, d.z);
-                kahan_add(un.w, c.w, d.w);
-                *reinterpret_cast<float4 *>(D.cmp[k] + q0) = c;
-            } else {
-                un.x += d.x; un.y += d.y; un.z += d.z; un.w += d.w;
+        for (int a = 0; a < NP; ++a) {
+            if (a >= a1) continue;
+            const long q = q0 + a;
+            const int e = edge_index(ix0 + a, iz, D.nx, D.nz, D.hpn);
+            const float m = D.mdt2[q];
+            float d[3], c[3] = {0.f, 0.f, 0.f}, ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) d[k] = D.del[k][q];
+            if (kahan) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) c[k] = D.cmp[k][q];
             }
-            *reinterpret_cast<float4 *>(D.del[k] + q0) = d;
-            *reinterpret_cast<float4 *>(D.u[p ^ 1][k] + q0) = un;
+            if (e >= 0) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) { ea[k] = D.edge_a[e * 3 + k]; eb[k] = D.edge_b[e * 3 + k]; }
+            }
+            float nu[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                d[k] = e < 0 ? d[k] - m * f[a][b][k] : ea[k] * d[k] - eb[k] * f[a][b][k];
+                nu[k] = u[a][b][k];
+                if (kahan) kahan_add(nu[k], c[k], d[k]);
+                else nu[k] = nu[k] + d[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                D.del[k][q] = d[k];
+                D.u[p ^ 1][k][q] = nu[k];
+                if (kahan) D.cmp[k][q] = c[k];
+            }
         }
     }
 }
@@ -581,31 +645,27 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     const bool last_x = ex == nex - 1;
     const bool direct = valid && (lane == 63 || last_x);
 
-    float Dg[NP][NP], Dw[NP][NP], cw[NP];
-#pragma unroll
-    for (int a = 0; a < NP; ++a) {
-        cw[a] = D.cw[a];
-#pragma unroll
-        for (int t = 0; t < NP; ++t) { Dg[a][t] = D.Dg[a][t]; Dw[a][t] = D.Dw[a][t]; }
-    }
     const float *__restrict__ U0 = FORCE_ONLY ? D.u[0][0] : D.u[p][0];
     const float *__restrict__ U1 = FORCE_ONLY ? D.u[0][1] : D.u[p][1];
     const float *__restrict__ U2 = FORCE_ONLY ? D.u[0][2] : D.u[p][2];
 
     float u[NP][NP][3];
     float f[NP][NP][3];
-    float cf[NP][3];
     int ids[NP][NP];
-#pragma unroll
-    for (int a = 0; a < NP; ++a)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) cf[a][i] = 0.f;
 
     const int ez0 = z0 > 0 ? z0 - 1 : 0;
     const int ez1 = z0 + rows;
     for (int ez = ez0; ez < ez1; ++ez) {
         const bool first = ez == ez0;
-        /* rows of u: row 0 comes from the previous element's top row */
+        /* rows of u: row 0 comes from the previous element's top row, and the
+           force accumulators start from the carried top-row contributions */
+#pragma unroll
+        for (int b = 0; b < NP; ++b) {
+#pragma unroll
+            for (int a = 0; a < NP; ++a)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) f[a][b][i] = (b == 0 && !first) ? f[a][N][i] : 0.f;
+        }
 #pragma unroll
         for (int b = 0; b < NP; ++b) {
             if (b == 0 && !first) {
@@ -639,7 +699,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
             float C[15];
 #pragma unroll
             for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
-            element_force<N, true>(u, f, Dg, Dw, cw, C, D.mat, ids);
+            element_force<N, true>(u, f, C, D.mat, ids);
         } else {
 #pragma unroll
             for (int b = 0; b < NP; ++b) {
@@ -648,7 +708,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
 #pragma unroll
                 for (int a = 0; a < NP; ++a) ids[a][b] = v[a];
             }
-            element_force<N, false>(u, f, Dg, Dw, cw, nullptr, D.mat, ids);
+            element_force<N, false>(u, f, nullptr, D.mat, ids);
         }
         if (!valid) {
 #pragma unroll
@@ -658,14 +718,6 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
 #pragma unroll
                     for (int i = 0; i < 3; ++i) f[a][b][i] = 0.f;
         }
-        /* assemble: bottom row gets the carried top row of the element below */
-#pragma unroll
-        for (int a = 0; a < NP; ++a)
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                f[a][0][i] += cf[a][i];
-                cf[a][i] = f[a][N][i];
-            }
         const bool top = ez == nez - 1;
         /* left column gets the right column of the lane to the left */
 #pragma unroll
@@ -678,27 +730,12 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
             }
         }
         if (own_x && ez >= z0) {
+            const int a1 = last_x ? NP : N;
 #pragma unroll
             for (int b = 0; b < NP; ++b) {
                 if (b == N && !top) continue;
                 const int iz = ez * N + b;
-                const long rowoff = (long)iz * ld;
-                const int ixb = ex * N;
-                if constexpr (N == 4) {
-                    update_row4<FORCE_ONLY>(D, p, rowoff + ixb, ixb, iz, f, u, b);
-                } else {
-#pragma unroll
-                    for (int a = 0; a < N; ++a) {
-                        float fa[3] = {f[a][b][0], f[a][b][1], f[a][b][2]};
-                        float ua[3] = {u[a][b][0], u[a][b][1], u[a][b][2]};
-                        update_node<FORCE_ONLY>(D, p, rowoff + ixb + a, ixb + a, iz, fa, ua);
-                    }
-                }
-                if (last_x) {
-                    float fa[3] = {f[N][b][0], f[N][b][1], f[N][b][2]};
-                    float ua[3] = {u[N][b][0], u[N][b][1], u[N][b][2]};
-                    update_node<FORCE_ONLY>(D, p, rowoff + ixb + N, ixb + N, iz, fa, ua);
-                }
+                update_row<N, FORCE_ONLY>(D, p, (long)iz * ld + ex * N, ex * N, iz, a1, f, u, b);
             }
         }
     }
@@ -887,7 +924,7 @@ static int persistent_blocks(const semsks_engine *e)
 static semsks_status launch_step(semsks_engine *e, hipEvent_t ta, hipEvent_t tb)
 {
     const int nslots = (int)e->slots.size();
-    hipLaunchKernelGGL(inject_kernel, dim3(nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
+    hipLaunchKernelGGL(inject_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
     if (ta) HIPCHK(hipEventRecord(ta, e->stream));
     const int nb = persistent_blocks(e);
     for (int n = 3; n <= 6; ++n) {
@@ -1257,7 +1294,7 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     HIPCHK(hipEventRecord(e->ev[0], e->stream));
     double kern_ms = 0.0;
     e->timed_steps = 0;
-    if (e->opts.use_graph) {
+    if (e->opts.use_graph && sync != 2) {
         const int G = 16;
         int done = 0;
         if (nsteps >= G) {

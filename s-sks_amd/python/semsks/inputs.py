@@ -81,6 +81,22 @@ def node_coords(nel: int, eh: float, hpn: int) -> np.ndarray:
     return e * eh + (xi[a] + 1.0) / 2.0 * eh
 
 
+def fit_bottom(top_layers, nez: int, eh: float) -> float:
+    """Bottom-layer thickness for which discret()'s float arithmetic yields
+    exactly nez elements of size eh (the total must sit just above nez*eh +
+    1e-3 after float rounding)."""
+    top = float_sum(top_layers)
+    want = nez * eh + 1.0e-3
+    x = F32(want - top)
+    for _ in range(4096):
+        tot = float_sum(list(top_layers) + [float(x)])
+        nz, e2, *_ = mesh_dims(tot, eh, 1, 4)
+        if nz == nez and e2 == float(F32(eh)):
+            return float(x)
+        x = np.nextafter(x, F32(np.inf))
+    raise ValueError("no float thickness gives nez=%d eh=%g" % (nez, eh))
+
+
 def float_sum(values) -> float:
     s = F32(0.0)
     for v in values:
@@ -143,7 +159,7 @@ def write_raymodel(path: str, thicknesses: list[float], vps, vss, rhos) -> None:
     with open(path, "w") as f:
         f.write("%d %d\n" % (jo, jo))
         for vp, vs, rho, th in zip(vps, vss, rhos, thicknesses):
-            f.write("%.4f %.4f %.4f %.4f\n" % (vp, vs, rho, th))
+            f.write("%.4f %.4f %.4f %.9g\n" % (vp, vs, rho, th))
         f.write("1\n")
         f.write("3 %d %d %d\n" % (jo - 1, jo, jo))   # nen=3, na[1] = nrec = jo
         f.write("1 1 1 1\n")
@@ -248,6 +264,12 @@ class Case:
     interfaces: list | None = None   # optional per-layer (xs, heights) overrides
     aniso: list | None = None        # optional per-layer (constants dict, dip6)
     extra: dict = field(default_factory=dict)
+    nez: int | None = None           # if set, the bottom layer is fitted to give exactly nez elements
+
+    def __post_init__(self):
+        if self.nez is not None:
+            top = list(self.layers_thk[:-1])
+            self.layers_thk = tuple(top) + (fit_bottom(top, self.nez, self.eh),)
 
     def derived(self):
         rng = np.random.default_rng(self.seed)
@@ -263,7 +285,7 @@ class Case:
         return dict(vp=vp, vs=vs, rho=rho, nez=nez, eh=eh, nx=nx, nz=nz, h=h, sdt=sdt,
                     thickness=thk)
 
-    def write(self, d: str, nt: int | None = None) -> dict:
+    def write(self, d: str, nt: int | None = None, write_records: bool = True) -> dict:
         os.makedirs(d, exist_ok=True)
         D = self.derived()
         vp, vs, rho = D["vp"], D["vs"], D["rho"]
@@ -299,11 +321,15 @@ class Case:
         # boundary records: dp = sdt, nt = nst + 3 (nt_sem = nst)
         nt = nt if nt is not None else self.nst + 3
         pw = PlaneWave(p=self.p, beta=vs[-1], f0=self.f0, psi_deg=self.psi_deg)
-        # record times use the float32 dp the solver reads
-        psv, sh = greenfile_records(pw, self.nex, D["nez"], D["eh"], self.hpn, self.neh, nt,
-                                    float(F32(sdt)))
-        if self.sh_zero:
-            sh[:] = 0
+        if write_records:
+            # record times use the float32 dp the solver reads
+            psv, sh = greenfile_records(pw, self.nex, D["nez"], D["eh"], self.hpn, self.neh, nt,
+                                        float(F32(sdt)))
+            if self.sh_zero:
+                sh[:] = 0
+        else:   # placeholders: the caller supplies the records another way
+            psv = np.zeros((nt, 2 * (D["nx"] + D["nz"])), F32)
+            sh = np.zeros((nt, D["nx"] + D["nz"]), F32)
         psv.tofile(os.path.join(d, "greenfile"))
         sh.tofile(os.path.join(d, "shgreenfile"))
         pars = dict(greenfile="greenfile", shgreenfile="shgreenfile", raymodel="raymodel",
