@@ -43,6 +43,10 @@ def load(kind: str = "f64") -> C.CDLL:
     L.orc_gll.argtypes = [C.c_int, vp, vp]
     L.orc_dlag.argtypes = [C.c_int, vp, vp]
     L.orc_set_node_tensor.argtypes = [vp, C.c_long, vp, C.c_float]
+    L.orc_interp_series.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_float, vp, vp]
+    L.orc_guard.argtypes = [vp, C.c_int]
+    L.orc_read_raymodel.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]
     _cache[kind] = L
     return L
 
@@ -118,8 +122,9 @@ class Oracle:
     def update(self, u, u0, G):
         def aos(a):
             return np.ascontiguousarray(np.moveaxis(a, 0, -1), dtype=self.dt)
-        u1 = np.zeros_like(aos(u))
-        self.L.orc_update(self.h, aos(u).ctypes.data, aos(u0).ctypes.data, aos(G).ctypes.data, u1.ctypes.data)
+        a, a0, g = aos(u), aos(u0), aos(G)      # keep the buffers alive across the call
+        u1 = np.zeros_like(a)
+        self.L.orc_update(self.h, a.ctypes.data, a0.ctypes.data, g.ctypes.data, u1.ctypes.data)
         return np.moveaxis(u1, -1, 0).copy()
 
     def boundary_series(self, n: int):
@@ -201,10 +206,11 @@ def ref_lib() -> C.CDLL | None:
     vp = C.c_void_p
     L.matrix.restype = C.POINTER(C.POINTER(C.c_float))
     L.matrix.argtypes = [C.c_long, C.c_long, C.c_long, C.c_long]
-    L.interpl.argtypes = [vp, C.c_long, C.c_int, C.c_float, C.c_int, C.POINTER(C.c_int),
+    # K&R definitions: float parameters arrive as promoted doubles
+    L.interpl.argtypes = [vp, C.c_long, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int),
                           C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_float), C.c_int, C.c_int]
     L.interp_const.restype = C.c_int
-    L.interp_const.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
+    L.interp_const.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
     L.read_model.argtypes = [C.c_int, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                              C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_float)),
                              C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_float)),
@@ -212,3 +218,37 @@ def ref_lib() -> C.CDLL | None:
                              C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.POINTER(C.c_int))]
     _cache["ref"] = L
     return L
+
+
+def interp_series(rec0, rec1, nsteps, rdt, kind="f32"):
+    """Raw 4-tap interpolation of in-memory records (one stream, or two
+    sharing the slot ring when rec1 is given)."""
+    L = load(kind)
+    dt = dtype_of(kind)
+    r0 = np.ascontiguousarray(rec0, np.float32)
+    r1 = None if rec1 is None else np.ascontiguousarray(rec1, np.float32)
+    o0 = np.zeros((nsteps, r0.shape[1]), dt)
+    o1 = None if r1 is None else np.zeros((nsteps, r1.shape[1]), dt)
+    st = L.orc_interp_series(r0.ctypes.data, r0.shape[1], None if r1 is None else r1.ctypes.data,
+                             0 if r1 is None else r1.shape[1], r0.shape[0], nsteps, rdt, o0.ctypes.data,
+                             None if o1 is None else o1.ctypes.data)
+    assert st == 0
+    return o0, o1
+
+
+def guard(g, kind="f32"):
+    L = load(kind)
+    a = np.ascontiguousarray(g, dtype_of(kind)).copy()
+    L.orc_guard(a.ctypes.data, a.size)
+    return a
+
+
+def read_raymodel(path, flat, kind="f32"):
+    L = load(kind)
+    cap = 64
+    arrs = [np.zeros(cap, np.float32) for _ in range(4)]
+    jo, nrec = C.c_int(), C.c_int()
+    st = L.orc_read_raymodel(path.encode(), flat, cap, *[a.ctypes.data for a in arrs], C.byref(jo), C.byref(nrec))
+    assert st == 0
+    n = jo.value
+    return dict(jo=n, nrec=nrec.value, cc=arrs[0][:n], ss=arrs[1][:n], dd=arrs[2][:n], tth=arrs[3][:n])

@@ -1098,6 +1098,54 @@ ORC_EXPORT int orc_boundary_series(const orc_ctx *c, int nsteps, real *g, real *
     return ORC_OK;
 }
 
+/* the raw interpolation (no guard) of one or two in-memory record streams,
+   stepping the slot ring exactly as the solvers do: one stream
+   (sem2d.c:410-413) or two sharing ns (sem2d_psvsh.c:430-431) */
+ORC_EXPORT int orc_interp_series(const float *rec0, int w0, const float *rec1, int w1, int nrec, int nsteps,
+                                 float rdt, real *out0, real *out1)
+{
+    orc_stream s[2];
+    int ns[4] = {0, 1, 2, 3}, it, i, k, nstream = rec1 ? 2 : 1;
+    const float *rec[2];
+    int w[2];
+    rec[0] = rec0; rec[1] = rec1; w[0] = w0; w[1] = w1;
+    for (k = 0; k < nstream; k++) {
+        s[k].n = w[k];
+        s[k].f = fmemopen((void *)rec[k], (size_t)nrec * w[k] * sizeof(float), "rb");
+        if (!s[k].f) return ORC_EIO;
+        s[k].sy = (float **)xcalloc(w[k], sizeof(float *));
+        for (i = 0; i < w[k]; i++) s[k].sy[i] = (float *)xcalloc(4, sizeof(float));
+    }
+    for (it = 0; it < nsteps; it++) {
+        interpl(&s[0], it, rdt, ns, out0 + (size_t)it * w0);
+        if (nstream == 2) interpl(&s[1], it, rdt, ns, out1 + (size_t)it * w1);
+    }
+    for (k = 0; k < nstream; k++) {
+        for (i = 0; i < w[k]; i++) free(s[k].sy[i]);
+        free(s[k].sy);
+        fclose(s[k].f);
+    }
+    return ORC_OK;
+}
+
+/* the guard on one vector (sem2d_psvsh.c:433-443) */
+ORC_EXPORT void orc_guard(real *g, int n) { guard(g, n); }
+
+/* read_model restated: returns jo and nrec, fills the (possibly flattened)
+   layer arrays (capacity cap) */
+ORC_EXPORT int orc_read_raymodel(const char *path, int flat, int cap, float *cc, float *ss, float *dd, float *tth,
+                                 int *jo, int *nrec)
+{
+    orc_raymodel m;
+    int rc = read_raymodel(path, flat, &m), j;
+    if (rc) return rc;
+    *jo = m.jo;
+    *nrec = m.na[m.nen[0] - 2];
+    for (j = 0; j < m.jo && j < cap; j++) { cc[j] = m.cc[j]; ss[j] = m.ss[j]; dd[j] = m.dd[j]; tth[j] = m.tth[j]; }
+    free_raymodel(&m);
+    return ORC_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* the time loop (sem2d_psvsh.c:418-512)                                     */
 /* traces: [nsteps][3][ntrace] in the order u, w(sign-flipped), v, i.e. the  */

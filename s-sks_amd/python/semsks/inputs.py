@@ -87,13 +87,20 @@ def fit_bottom(top_layers, nez: int, eh: float) -> float:
     1e-3 after float rounding)."""
     top = float_sum(top_layers)
     want = nez * eh + 1.0e-3
-    x = F32(want - top)
+    x0 = F32(want - top)
+    best = None
+    x = x0
     for _ in range(4096):
         tot = float_sum(list(top_layers) + [float(x)])
         nz, e2, *_ = mesh_dims(tot, eh, 1, 4)
-        if nz == nez and e2 == float(F32(eh)):
-            return float(x)
+        if nz == nez:
+            if e2 == float(F32(eh)):
+                return float(x)
+            if best is None:
+                best = float(x)       # eh one float step away from the request
         x = np.nextafter(x, F32(np.inf))
+    if best is not None:
+        return best
     raise ValueError("no float thickness gives nez=%d eh=%g" % (nez, eh))
 
 

@@ -1,0 +1,300 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each oracle function is checked against something that does not restate it:
+  * golden vectors written by the reference's own interpl.c / readmodel.c
+    (tests/golden/make_ref_vectors.py, via oracle/_ref);
+  * closed forms (GLL nodes and weights, Lagrange derivative identities);
+  * an independent FP64 assembly of K from explicit strain-displacement
+    matrices on tiny meshes;
+  * invariants: symmetry and null space of K, the patch test, mass = integral
+    of rho, discrete energy conservation of the explicit update;
+  * hand-traceable rules of the reference (interpolation delays at rdt=1,
+    the sequential guard, receiver indexing, the nez/eh float arithmetic).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+from semsks import inputs
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------- GLL tables
+CLOSED = {
+    3: ([-1, -math.sqrt(1 / 5), math.sqrt(1 / 5), 1], [1 / 6, 5 / 6, 5 / 6, 1 / 6]),
+    4: ([-1, -math.sqrt(3 / 7), 0, math.sqrt(3 / 7), 1], [1 / 10, 49 / 90, 32 / 45, 49 / 90, 1 / 10]),
+    5: ([-1, -math.sqrt(1 / 3 + 2 * math.sqrt(7) / 21), -math.sqrt(1 / 3 - 2 * math.sqrt(7) / 21),
+         math.sqrt(1 / 3 - 2 * math.sqrt(7) / 21), math.sqrt(1 / 3 + 2 * math.sqrt(7) / 21), 1],
+        [1 / 15, (14 - math.sqrt(7)) / 30, (14 + math.sqrt(7)) / 30, (14 + math.sqrt(7)) / 30,
+         (14 - math.sqrt(7)) / 30, 1 / 15]),
+}
+
+
+@pytest.mark.parametrize("hpn", [3, 4, 5])
+def test_gll_closed_forms(hpn):
+    x, w, _ = pyoracle.gll(hpn, "f64")
+    xc, wc = map(np.array, CLOSED[hpn])
+    assert np.allclose(x, xc, rtol=0, atol=2e-15)
+    assert np.allclose(w, wc, rtol=0, atol=2e-15)
+    x32, w32, _ = pyoracle.gll(hpn, "f32")
+    assert np.all(np.abs(x32 - xc.astype(np.float32)) <= np.spacing(np.float32(1)))
+    assert np.all(np.abs(w32 - wc) <= 2 * np.spacing(np.float32(1)))
+
+
+@pytest.mark.parametrize("hpn", [3, 4, 5, 6])
+def test_gll_quadrature_exactness(hpn):
+    x, w, _ = pyoracle.gll(hpn, "f64")
+    assert abs(w.sum() - 2) < 1e-14
+    for k in range(2 * hpn):            # exact for degree <= 2N-1
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        assert abs(np.dot(w, x ** k) - exact) < 1e-13
+
+
+@pytest.mark.parametrize("hpn", [3, 4, 5, 6])
+def test_derivative_matrix_identities(hpn):
+    x, w, d = pyoracle.gll(hpn, "f64")      # d[j][i] = l'_i(x_j)
+    # exact derivative of every polynomial of degree <= N
+    for k in range(hpn + 1):
+        assert np.allclose(d @ x ** k, k * x ** max(k - 1, 0) if k else 0 * x, atol=1e-12)
+    assert np.allclose(d.sum(axis=1), 0, atol=1e-13)
+    n = hpn
+    assert abs(d[0, 0] + n * (n + 1) / 4) < 1e-12 and abs(d[n, n] - n * (n + 1) / 4) < 1e-12
+    # independent: derivative of the Lagrange basis via numpy polynomials
+    for i in range(n + 1):
+        others = [x[m] for m in range(n + 1) if m != i]
+        P = np.poly1d(np.poly(others)) / np.prod([x[i] - o for o in others])
+        assert np.allclose(P.deriv()(x), d[:, i], atol=1e-12)
+
+
+# ---------------------------------------------------------------- golden: interpl
+def test_interpolation_matches_reference_interpl():
+    g = np.load(os.path.join(GOLD, "ref_interpl.npz"))
+    psv, sh = g["psv"], g["sh"]
+    for name in ("r1_single", "r07_single", "r1p_single", "r1_shared", "r07_shared", "r037_shared"):
+        rdt = float(g[name + "_rdt"])
+        shared = name.endswith("shared")
+        o0, o1 = pyoracle.interp_series(psv, sh if shared else None, 40, rdt, "f32")
+        assert np.array_equal(o0.view(np.uint32), g[name + "_g"].view(np.uint32)), name
+        if shared:
+            assert np.array_equal(o1.view(np.uint32), g[name + "_gv"].view(np.uint32)), name
+
+
+def test_interpolation_delays_at_rdt1():
+    """With dp = sdt, the single-stream solver injects record it-2 and the
+    shared-ring solver record it-1 (interpl.c:59-91 with sem2d.c:410-413 /
+    sem2d_psvsh.c:430-431)."""
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal((30, 7)).astype(np.float32)
+    s = rng.standard_normal((30, 5)).astype(np.float32)
+    one, _ = pyoracle.interp_series(r, None, 20, 1.0, "f32")
+    assert np.array_equal(one[2:], r[:18]) and not one[:2].any()
+    a, b = pyoracle.interp_series(r, s, 20, 1.0, "f32")
+    assert np.array_equal(a[1:], r[:19]) and np.array_equal(b[1:], s[:19]) and not a[0].any()
+
+
+def test_guard_sequential_rule():
+    """Non-finite or |g|>=10 values become the mean of the already-repaired
+    left neighbour and the original right neighbour (sem2d_psvsh.c:433-443)."""
+    g = pyoracle.guard(np.array([1, 20, 30, 5, np.nan, 7, 9.99, -10, 2], np.float32))
+    assert g[1] == np.float32((1 + 30) / 2)          # 15.5, not re-checked
+    assert g[2] == np.float32((15.5 + 5) / 2)        # uses the repaired g[1]
+    assert g[4] == np.float32((5 + 7) / 2)
+    assert g[6] == np.float32(9.99)
+    assert g[7] == np.float32((9.99 + 2) / 2)
+
+
+# ---------------------------------------------------------------- golden: read_model
+def test_raymodel_reader_matches_reference(tmp_path):
+    g = np.load(os.path.join(GOLD, "ref_readmodel.npz"))
+    for name in ("flat3", "sph3", "rays2"):
+        p = tmp_path / name
+        p.write_text(str(g[name + "_text"]))
+        r = pyoracle.read_raymodel(os.fspath(p), int(g[name + "_flat"]))
+        assert r["jo"] == int(g[name + "_jo"]) and r["nrec"] == int(g[name + "_nrec"]), name
+        for k in ("cc", "ss", "dd", "tth"):
+            assert np.array_equal(r[k].view(np.uint32), g[name + "_" + k].view(np.uint32)), (name, k)
+
+
+# ---------------------------------------------------------------- mesh + receivers
+def test_mesh_dims_float_arithmetic(tmp_path):
+    """nez=(int)((thk-1e-3)/eh) with a float thickness: 200.001 gives 100
+    elements of 2.0 km, 24.001 gives 11 (its float value is below 24.001)."""
+    assert inputs.mesh_dims(200.001, 2.0, 1, 4)[:2] == (100, 2.0)
+    assert inputs.mesh_dims(24.001, 2.0, 1, 4)[0] == 11
+    for thk, nez in ((24.001, 11), (24.002, 12)):
+        c = inputs.small_case(layers_thk=(4.0, 4.0, thk - 8.0), nst=3)
+        d = tmp_path / ("m%d" % nez)
+        c.write(os.fspath(d))
+        o = pyoracle.Oracle(["par=run.par"], "f32", cwd=os.fspath(d))
+        assert o.nez == inputs.mesh_dims(inputs.float_sum([4.0, 4.0, thk - 8.0]), 2.0, 1, 4)[0]
+
+
+def test_receiver_indexing(tmp_path):
+    c = inputs.small_case(nst=3)
+    d = os.fspath(tmp_path)
+    D = c.write(d)
+    with open(os.path.join(d, "rec_file"), "w") as f:
+        f.write("4\n12.5000\n12.4999\n0.0000\n47.9999\n")
+    o = pyoracle.Oracle(["par=run.par"], "f32", cwd=d)
+    ix, iz = o.receivers()
+    h = np.float32(D["h"])
+    expect = [int(np.float32(x) / h) for x in (12.5, 12.4999, 0.0, 47.9999)]
+    assert list(ix) == expect and all(iz == o.nz - 1)
+    o2 = pyoracle.Oracle(["par=run.par", "get_rec=0", "xbeg=10", "idxplot=3", "xtrace=5"], "f32", cwd=d)
+    ix2, _ = o2.receivers()
+    assert list(ix2) == [int(np.float32(10) / h - 1) + 3 * k for k in range(5)]
+
+
+# ---------------------------------------------------------------- element force
+def brute_force_K(o):
+    """Independent global K in FP64: explicit strain-displacement matrices
+    at the collocated GLL quadrature points, K_e = sum_q w_q Je B^T C B."""
+    x, w, _ = pyoracle.gll(o.hpn, "f64")
+    n = o.hpn
+    D = np.zeros((n + 1, n + 1))
+    for i in range(n + 1):
+        others = [x[m] for m in range(n + 1) if m != i]
+        P = np.poly1d(np.poly(others)) / np.prod([x[i] - q for q in others])
+        D[:, i] = P.deriv()(x)
+    eh = o.eh
+    je, jinv = eh * eh / 4.0, 2.0 / eh
+    mat = o.material()
+    N = o.nx * o.nz
+    K = np.zeros((3 * N, 3 * N))
+    for ex in range(o.nex):
+        for ez in range(o.nez):
+            nodes = [(ez * n + b) * o.nx + ex * n + a for b in range(n + 1) for a in range(n + 1)]
+            idx = np.array([[3 * q + c for c in range(3)] for q in nodes]).ravel()
+            Ke = np.zeros((len(idx), len(idx)))
+            for b in range(n + 1):
+                for a in range(n + 1):
+                    B = np.zeros((5, len(idx)))     # rows: e_xx, e_zz, e_yz, e_xz(eng), e_xy(eng)
+                    for t in range(n + 1):
+                        kx = (b * (n + 1) + t) * 3          # node (t, b): d/dx
+                        kz = (t * (n + 1) + a) * 3          # node (a, t): d/dz
+                        B[0, kx + 0] += D[a, t] * jinv
+                        B[4, kx + 1] += D[a, t] * jinv
+                        B[3, kx + 2] += D[a, t] * jinv
+                        B[1, kz + 2] += D[b, t] * jinv
+                        B[2, kz + 1] += D[b, t] * jinv
+                        B[3, kz + 0] += D[b, t] * jinv
+                    r = mat[nodes[b * (n + 1) + a]]
+                    C = np.zeros((5, 5))
+                    vals = r[1:]
+                    k = 0
+                    for i in range(5):
+                        for j in range(i, 5):
+                            C[i, j] = C[j, i] = vals[k]
+                            k += 1
+                    Ke += w[a] * w[b] * je * B.T @ C @ B
+            K[np.ix_(idx, idx)] += Ke
+    return K
+
+
+def tiny_oracle(tmp_path, aniso_layer=None, nex=3, nez=3, homogeneous=False):
+    from semsks import aniso
+    an = None
+    if aniso_layer:
+        an = [aniso.hexagonal_tilted(7.8, 4.4, 3.3, 0.05, 35.0, 70.0), None, None]
+    kw = dict(interfaces=[([0.0, 8.0], [1.3, 3.9]), None, None], aniso=an)
+    if homogeneous:
+        kw = dict(vp=(7.0,) * 3, vs=(4.0,) * 3, rho=(3.0,) * 3, jitter=0.0)
+    c = inputs.Case(nex=nex, nez=nez, eh=2.0, layers_thk=(1.5, 1.7, 4.0), nst=3, nrec=2, **kw)
+    d = os.fspath(tmp_path)
+    c.write(d)
+    return pyoracle.Oracle(["par=run.par"], "f64", cwd=d)
+
+
+@pytest.mark.parametrize("tilted", [False, True])
+def test_stiffness_matches_independent_assembly(tmp_path, tilted):
+    o = tiny_oracle(tmp_path, tilted)
+    K = brute_force_K(o)
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal((3, o.nz, o.nx))
+    f = o.stiff(u)
+    fa = K @ np.moveaxis(u, 0, -1).ravel()
+    assert np.allclose(np.moveaxis(f, 0, -1).ravel(), fa, rtol=0, atol=1e-11 * np.abs(fa).max())
+    # K symmetric and positive semi-definite
+    assert np.allclose(K, K.T, atol=1e-12 * np.abs(K).max())
+    assert np.linalg.eigvalsh(K).min() > -1e-9 * np.abs(K).max()
+
+
+def test_stiffness_null_space_and_patch(tmp_path):
+    o = tiny_oracle(tmp_path, True, nex=4, nez=4)
+    X = np.arange(o.nx)[None, :] * 1.0
+    Z = np.arange(o.nz)[:, None] * 1.0
+    from semsks.inputs import node_coords
+    xs = node_coords(o.nex, o.eh, o.hpn)[None, :] + 0 * Z
+    zs = node_coords(o.nez, o.eh, o.hpn)[:, None] + 0 * X
+    one = np.ones((o.nz, o.nx))
+    for u in (np.stack([one, 0 * one, 0 * one]), np.stack([0 * one, one, 0 * one]),
+              np.stack([0 * one, 0 * one, one]), np.stack([zs, 0 * one, -xs])):   # translations, rotation about y
+        f = o.stiff(u)
+        assert np.abs(f).max() < 1e-10
+    # patch test: a linear field has constant strain, so in a homogeneous
+    # model every node off the domain edges carries no force, and the edge
+    # forces are the tractions of the constant stress
+    o2 = tiny_oracle(tmp_path / "h", False, nex=4, nez=4, homogeneous=True)
+    u = np.stack([0.3 * xs + 0.1 * zs, -0.2 * xs + 0.05 * zs, 0.1 * xs - 0.4 * zs])
+    f = o2.stiff(u)
+    assert np.abs(f[:, 1:-1, 1:-1]).max() < 1e-10 * np.abs(f).max()
+    assert np.abs(f[:, 0, :]).max() > 1e-3 * np.abs(f).max()
+    # total edge force balances (zero net force of a self-equilibrated state)
+    assert np.abs(f.sum(axis=(1, 2))).max() < 1e-10 * np.abs(f).max()
+
+
+# ---------------------------------------------------------------- mass, update
+def test_mass_integrates_density(tmp_path):
+    c = inputs.small_case(vp=(6.0, 6.0, 6.0), vs=(3.5, 3.5, 3.5), rho=(2.7, 2.7, 2.7), jitter=0.0, nst=3)
+    d = os.fspath(tmp_path)
+    c.write(d)
+    o = pyoracle.Oracle(["par=run.par"], "f64", cwd=d)
+    m, md, ab = o.mass()
+    area = (o.nex * o.eh) * (o.nez * o.eh)
+    assert abs(m.sum() - float(np.float32(2.7)) * area) < 1e-12 * m.sum()
+    assert np.allclose(md, np.float32(o.sdt) ** 2 / m.astype(np.float64), rtol=1e-15)
+    # Lysmer coefficients live on the left, bottom and right edges only
+    inner = ab.copy()
+    inner[:, 0] = inner[0, :] = inner[:, -1] = 0
+    assert not inner.any() and ab[0, :].all() and ab[1:, 0].all() and ab[1:, -1].all()
+    # bottom normal (z) uses vp, tangential vs: ratio vp/vs at an edge node
+    k = ab[0, 5]
+    assert abs(k[2] / k[0] - 6.0 / 3.5) < 1e-6
+
+
+def test_energy_conserved_without_absorption(tmp_path):
+    """Central differences conserve E = 1/2 v'Mv + 1/2 u(n+1)'K u(n)."""
+    c = inputs.small_case(nst=3, absorb=0)
+    d = os.fspath(tmp_path)
+    c.write(d)
+    o = pyoracle.Oracle(["par=run.par"], "f64", cwd=d)
+    m, _, _ = o.mass()
+    zs, xs = np.meshgrid(np.linspace(0, 1, o.nz), np.linspace(0, 1, o.nx), indexing="ij")
+    bump = np.exp(-((xs - 0.5) ** 2 + (zs - 0.5) ** 2) / 0.01)
+    u0 = np.stack([bump, 0.3 * bump, -0.5 * bump])
+    um = u0.copy()
+    E = []
+    for _ in range(150):
+        G = o.stiff(u0)
+        u1 = o.update(u0, um, G)
+        v = (u1 - u0) / o.sdt
+        E.append(0.5 * np.sum(m * v * v) + 0.5 * np.sum(u1 * o.stiff(u0)))
+        um, u0 = u0, u1
+    E = np.array(E)
+    assert np.abs(E - E[0]).max() < 1e-10 * abs(E[0])
+
+
+def test_faithful_fp32_tracks_fp64(tmp_path):
+    c = inputs.small_case(nst=300, psi_deg=30.0, sh_zero=False)
+    d = os.fspath(tmp_path)
+    c.write(d)
+    t32 = pyoracle.Oracle(["par=run.par"], "f32", cwd=d).run()
+    t64 = pyoracle.Oracle(["par=run.par"], "f64", cwd=d).run()
+    ref = max(np.linalg.norm(t64[:, k]) for k in range(3))
+    for k in range(3):
+        scale = max(np.linalg.norm(t64[:, k]), 1e-3 * ref)
+        assert np.linalg.norm(t32[:, k] - t64[:, k]) <= 1e-5 * scale
