@@ -114,7 +114,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--kernel-steps", type=int, default=10)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run one device copy of this many MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
               file=sys.stderr)

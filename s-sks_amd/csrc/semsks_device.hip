@@ -150,7 +150,40 @@ __global__ void mass_kernel(InstDesc D, const double *w64, double je, float sdt,
         double t = w64[m] * w64[k] * je * rho, s = 0.0;
         for (int c = 0; c < cx * cz; ++c) s = s + t;
         if (mass) mass[q] = s;
-        ((float *)D.mdt2)[(long)iz * D.ld + ix] = (float)((double)sdt * (double)sdt / s);
+    }
+}
+
+/* dt^2/M per material record and position class (cz, cx): class 0 = node
+   shared by two elements (two equal terms w_0), 1..N-1 = interior GLL
+   point, N = first/last node of the mesh (one term w_0 = w_N).  The sum is
+   formed term by term as in Mass_Matrix, so every entry equals dt^2/M of
+   every node of that class, bit for bit. */
+__global__ void mclass_kernel(const float *mat, int n_mat, int n, const double *w64, double je, float sdt, float *out)
+{
+    const int NC = n + 1;
+    for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < n_mat * NC * NC; id += gridDim.x * blockDim.x) {
+        const int m = id / (NC * NC), r = id % (NC * NC), cz = r / NC, cx = r % NC;
+        const int mx = cx == n ? 0 : cx, mz = cz == n ? 0 : cz;
+        const int cnt = (cx == 0 ? 2 : 1) * (cz == 0 ? 2 : 1);
+        const double rho = (double)mat[(long)m * SEMSKS_MAT_STRIDE];
+        const double t = w64[mx] * w64[mz] * je * rho;
+        double s = 0.0;
+        for (int c = 0; c < cnt; ++c) s = s + t;
+        out[id] = (float)((double)sdt * (double)sdt / s);
+    }
+}
+
+/* per-node dt^2/M from the class table (for inspection) */
+__global__ void expand_mdt2_kernel(InstDesc D, float *out)
+{
+    const int n = D.hpn, NC = n + 1;
+    const long N = (long)D.nx * D.nz;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < N; q += (long)gridDim.x * blockDim.x) {
+        const int iz = (int)(q / D.nx), ix = (int)(q % D.nx);
+        const int cx = (ix == 0 || ix == D.nx - 1) ? n : ix % n;
+        const int cz = (iz == 0 || iz == D.nz - 1) ? n : iz % n;
+        const int mid = D.mat_id[(long)iz * D.ld + ix];
+        out[q] = D.mclass[(long)mid * NC * NC + cz * NC + cx];
     }
 }
 
@@ -287,6 +320,29 @@ __device__ __forceinline__ void kahan_add(float &u, float &c, float x)
     u = t;
 }
 
+/* The Kahan compensation c of u is the rounding error of the last addition,
+   |c| <= ulp(u)/2.  It is stored per component as a signed byte in units of
+   ulp(u)/256, three components per 32-bit word; the residual it leaves is
+   below ulp(u)/512 per step. */
+__device__ __forceinline__ int c_exponent(float u)
+{
+    const int e = (int)((__float_as_uint(u) >> 23) & 0xffu);
+    return e > 0 ? e : 1;
+}
+
+__device__ __forceinline__ float c_unpack(uint32_t w, int k, float u)
+{
+    const int q = (int)(int8_t)((w >> (8 * k)) & 0xffu);
+    return ldexpf((float)q, c_exponent(u) - 158);
+}
+
+__device__ __forceinline__ uint32_t c_pack(float c, float u)
+{
+    int q = (int)rintf(ldexpf(c, 158 - c_exponent(u)));
+    q = q > 127 ? 127 : (q < -127 ? -127 : q);
+    return (uint32_t)q & 0xffu;
+}
+
 /* GRT_Interface4: u += g(t) - g(t-dt) (w sign flipped); GRT_Interface3:
    u += -u(t-dt) + g(t).  Both change u(t) in place before the force, and the
    reference then copies the changed u(t) into u(t-dt), so delta moves with u. */
@@ -332,18 +388,21 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
                 d[2] = g0[2 * nx + nz + iz] - g[2 * nx + nz + iz];
             }
         }
+        const uint32_t w = D.kahan ? D.cmp[q] : 0u;
+        uint32_t wn = 0;
         for (int k = 0; k < 3; ++k) {
             float u = U[k][q];
             if (D.kahan) {
-                float c = D.cmp[k][q];
+                float c = c_unpack(w, k, u);
                 kahan_add(u, c, d[k]);
-                D.cmp[k][q] = c;
+                wn |= c_pack(c, u) << (8 * k);
             } else {
                 u = u + d[k];
             }
             U[k][q] = u;
             D.del[k][q] = D.del[k][q] + d[k];
         }
+        if (D.kahan) D.cmp[q] = wn;
     }
 }
 
@@ -433,38 +492,6 @@ __device__ __forceinline__ constexpr float opDw(int a, int t) { constexpr ElemOp
 template <int N>
 __device__ __forceinline__ constexpr float opW(int a) { constexpr ElemOps<N> o{}; return o.w[a]; }
 
-template <int N>
-__device__ __forceinline__ void load_cols(const float *__restrict__ row, int ex, bool valid, bool direct_last,
-                                          float (&v)[N + 1])
-{
-    if (valid) {
-        if constexpr (N == 4) {
-            float4 t = *reinterpret_cast<const float4 *>(row + 4 * ex);
-            v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-        } else {
-#pragma unroll
-            for (int a = 0; a < N; ++a) v[a] = row[N * ex + a];
-        }
-    } else {
-#pragma unroll
-        for (int a = 0; a < N; ++a) v[a] = 0.f;
-    }
-    float nb = __shfl_down(v[0], 1, 64);
-    if (direct_last) nb = row[N * ex + N];
-    v[N] = nb;
-}
-
-template <int N>
-__device__ __forceinline__ void load_ids(const uint16_t *__restrict__ row, int ex, bool valid, bool direct_last,
-                                         int (&v)[N + 1])
-{
-#pragma unroll
-    for (int a = 0; a < N; ++a) v[a] = valid ? (int)row[N * ex + a] : 0;
-    int nb = __shfl_down(v[0], 1, 64);
-    if (direct_last) nb = row[N * ex + N];
-    v[N] = nb;
-}
-
 /* stress at one node from the 5x5 block C over {xx, zz, yz, xz, xy} and the
    2-D strains (uy,y = ux,y = uz,y = 0, Stress_ij sem2d_psvsh.c:1345-1379);
    C: 0:00 1:02 2:03 3:04 4:05 5:22 6:23 7:24 8:25 9:33 10:34 11:35 12:44 13:45 14:55 */
@@ -478,60 +505,6 @@ __device__ __forceinline__ void stress5(const float *C, float e0, float e2, floa
     s5 = C[4] * e0 + C[8] * e2 + C[11] * e3 + C[13] * e4 + C[14] * e5;
 }
 
-/* K_e u_e of one element (Stiff_Matrix's dUj_di_2D -> Stress_ij -> F_ik ->
-   Grid_Stiff), accumulated onto f (whose bottom row may already hold the
-   carried top row of the element below) */
-template <int N, bool UNIFORM>
-__device__ __forceinline__ void element_force(const float (&u)[N + 1][N + 1][3], float (&f)[N + 1][N + 1][3],
-                                              const float *Cu, const float *__restrict__ mat,
-                                              const int (&ids)[N + 1][N + 1])
-{
-    constexpr int NP = N + 1;
-#pragma unroll
-    for (int b = 0; b < NP; ++b) {
-#pragma unroll
-        for (int a = 0; a < NP; ++a) {
-            float gx0 = 0.f, gx1 = 0.f, gx2 = 0.f, gz0 = 0.f, gz1 = 0.f, gz2 = 0.f;
-#pragma unroll
-            for (int t = 0; t < NP; ++t) {
-                const float dx = opD<N>(a, t), dz = opD<N>(b, t);
-                gx0 += dx * u[t][b][0];
-                gx1 += dx * u[t][b][1];
-                gx2 += dx * u[t][b][2];
-                gz0 += dz * u[a][t][0];
-                gz1 += dz * u[a][t][1];
-                gz2 += dz * u[a][t][2];
-            }
-            float C[15];
-            if constexpr (UNIFORM) {
-#pragma unroll
-                for (int k = 0; k < 15; ++k) C[k] = Cu[k];
-            } else {
-                const float4 *r = reinterpret_cast<const float4 *>(mat + (long)ids[a][b] * SEMSKS_MAT_STRIDE);
-                float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-                C[0] = r0.y; C[1] = r0.z; C[2] = r0.w; C[3] = r1.x; C[4] = r1.y; C[5] = r1.z; C[6] = r1.w;
-                C[7] = r2.x; C[8] = r2.y; C[9] = r2.z; C[10] = r2.w; C[11] = r3.x; C[12] = r3.y; C[13] = r3.z;
-                C[14] = r3.w;
-            }
-            float s0, s2, s3, s4, s5;
-            stress5(C, gx0, gz2, gz1, gx2 + gz0, gx1, s0, s2, s3, s4, s5);
-            const float wb = opW<N>(b), wa = opW<N>(a);
-            const float p0 = wb * s0, p1 = wb * s5, p2 = wb * s4;   /* row traction    T_i0 */
-            const float q0 = wa * s4, q1 = wa * s3, q2 = wa * s2;   /* column traction T_i2 */
-#pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                const float ax = opDw<N>(a, k), az = opDw<N>(b, k);
-                f[k][b][0] += ax * p0;
-                f[k][b][1] += ax * p1;
-                f[k][b][2] += ax * p2;
-                f[a][k][0] += az * q0;
-                f[a][k][1] += az * q1;
-                f[a][k][2] += az * q2;
-            }
-        }
-    }
-}
-
 __device__ __forceinline__ int edge_index(int ix, int iz, int nx, int nz, int n)
 {
     if (iz == 0) return ix;
@@ -541,212 +514,415 @@ __device__ __forceinline__ int edge_index(int ix, int iz, int nx, int nz, int n)
     return -1;
 }
 
-/* update of the owned nodes (columns a0..a1-1) of one element row.  All
-   loads are issued before any store: the state planes are distinct arrays,
-   but the compiler cannot prove it, so interleaving would serialise the
-   memory round trips. */
-template <int N, bool FORCE_ONLY>
-__device__ __forceinline__ void update_row(const InstDesc &D, int p, long q0, int ix0, int iz, int a1,
-                                           const float (&f)[N + 1][N + 1][3], const float (&u)[N + 1][N + 1][3],
-                                           int b)
+/* dt^2/M of a node from its material record and its position class: the
+   lumped mass is rho * Je * Wx * Wz with Wx the GLL weight of the node's
+   column inside its element(s) -- class 0: shared by two elements, 1..N-1:
+   interior, N: the domain's first or last column (one element) */
+template <int N>
+__device__ __forceinline__ int mass_class(int i, int n_last)
 {
-    constexpr int NP = N + 1;
-    if constexpr (FORCE_ONLY) {
-#pragma unroll
-        for (int a = 0; a < NP; ++a)
-            if (a < a1)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) D.fout[k][q0 + a] = f[a][b][k];
-        return;
-    } else {
-        const bool kahan = D.kahan != 0;
-        const bool edge_row = (iz == 0) || (iz == D.nz - 1) || (ix0 == 0) || (a1 == NP);
-        if constexpr (N == 4) {
-            if (!edge_row) {
-                const float4 m = *reinterpret_cast<const float4 *>(D.mdt2 + q0);
-                float4 d[3], c[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) d[k] = *reinterpret_cast<const float4 *>(D.del[k] + q0);
-                if (kahan) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) c[k] = *reinterpret_cast<const float4 *>(D.cmp[k] + q0);
-                }
-                float4 un[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    d[k].x = d[k].x - m.x * f[0][b][k];
-                    d[k].y = d[k].y - m.y * f[1][b][k];
-                    d[k].z = d[k].z - m.z * f[2][b][k];
-                    d[k].w = d[k].w - m.w * f[3][b][k];
-                    un[k] = make_float4(u[0][b][k], u[1][b][k], u[2][b][k], u[3][b][k]);
-                    if (kahan) {
-                        kahan_add(un[k].x, c[k].x, d[k].x);
-                        kahan_add(un[k].y, c[k].y, d[k].y);
-                        kahan_add(un[k].z, c[k].z, d[k].z);
-                        kahan_add(un[k].w, c[k].w, d[k].w);
-                    } else {
-                        un[k].x += d[k].x; un[k].y += d[k].y; un[k].z += d[k].z; un[k].w += d[k].w;
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    *reinterpret_cast<float4 *>(D.del[k] + q0) = d[k];
-                    *reinterpret_cast<float4 *>(D.u[p ^ 1][k] + q0) = un[k];
-                    if (kahan) *reinterpret_cast<float4 *>(D.cmp[k] + q0) = c[k];
-                }
-                return;
-            }
-        }
-        /* general path (edge rows/columns, the domain's last column, N != 4):
-           one node at a time, its loads still ahead of its stores */
-#pragma unroll
-        for (int a = 0; a < NP; ++a) {
-            if (a >= a1) continue;
-            const long q = q0 + a;
-            const int e = edge_index(ix0 + a, iz, D.nx, D.nz, D.hpn);
-            const float m = D.mdt2[q];
-            float d[3], c[3] = {0.f, 0.f, 0.f}, ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) d[k] = D.del[k][q];
-            if (kahan) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) c[k] = D.cmp[k][q];
-            }
-            if (e >= 0) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) { ea[k] = D.edge_a[e * 3 + k]; eb[k] = D.edge_b[e * 3 + k]; }
-            }
-            float nu[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                d[k] = e < 0 ? d[k] - m * f[a][b][k] : ea[k] * d[k] - eb[k] * f[a][b][k];
-                nu[k] = u[a][b][k];
-                if (kahan) kahan_add(nu[k], c[k], d[k]);
-                else nu[k] = nu[k] + d[k];
-            }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                D.del[k][q] = d[k];
-                D.u[p ^ 1][k][q] = nu[k];
-                if (kahan) D.cmp[k][q] = c[k];
-            }
-        }
-    }
+    if (i == 0 || i == n_last) return N;
+    const int r = i % N;
+    return r;
 }
 
-template <int N, bool FORCE_ONLY>
-__device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows, int lane)
+template <int N>
+__device__ __forceinline__ float mdt2_of(const InstDesc &D, int mid, int ix, int iz)
+{
+    constexpr int NC = N + 1;
+    return gp(D.mclass)[(long)mid * NC * NC + mass_class<N>(iz, D.nz - 1) * NC + mass_class<N>(ix, D.nx - 1)];
+}
+
+/* one node: delta, Kahan-compensated u, edge rows by their Lysmer factors */
+template <int N>
+__device__ __forceinline__ void update_one(const InstDesc &D, int p, long q, int ix, int iz, int mid,
+                                           const float (&f)[3], const float (&uc)[3])
+{
+    const bool kahan = D.kahan != 0;
+    const int e = edge_index(ix, iz, D.nx, D.nz, D.hpn);
+    float d[3], ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = gp(D.del[k])[q];
+    const uint32_t w = kahan ? gp(D.cmp)[q] : 0u;
+    float m = 0.f;
+    if (e >= 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { ea[k] = gp(D.edge_a)[e * 3 + k]; eb[k] = gp(D.edge_b)[e * 3 + k]; }
+    } else {
+        m = mdt2_of<N>(D, mid, ix, iz);
+    }
+    float nu[3];
+    uint32_t wn = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        d[k] = e < 0 ? d[k] - m * f[k] : ea[k] * d[k] - eb[k] * f[k];
+        nu[k] = uc[k];
+        if (kahan) {
+            float c = c_unpack(w, k, nu[k]);
+            kahan_add(nu[k], c, d[k]);
+            wn |= c_pack(c, nu[k]) << (8 * k);
+        } else {
+            nu[k] = nu[k] + d[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gp(D.del[k])[q] = d[k];
+        gp(D.u[p ^ 1][k])[q] = nu[k];
+    }
+    if (kahan) gp(D.cmp)[q] = wn;
+}
+
+/* Lane layout of the element-force/update kernel: a 64-lane wave holds
+   NE = 64/(N+1) element columns side by side, N+1 lanes per element, one
+   lane per element ROW (lane = e*(N+1) + b).  A lane keeps only its row of u
+   and of the element force in registers; derivatives and tractions along x
+   stay inside the lane, the ones along z go through small per-wave LDS
+   blocks.  The wave sweeps its tile upward one element row per iteration. */
+template <int N> struct RowLayout {
+    static constexpr int NP = N + 1;
+    static constexpr int NE = 64 / NP;              /* element columns per wave */
+    static constexpr int NL = NE * NP;              /* active lanes */
+    static constexpr int XCHUNK = NE - 1;           /* owned columns per tile (one halo column) */
+    static constexpr int CB = (3 * NP + 3) / 4 * 4; /* floats per column block [k][t], 16-byte multiple */
+    static constexpr int ESTR = NP * CB + 4;        /* floats per element block (u or q) */
+    /* floats of a carried force row; an odd number of 16-byte groups keeps
+       the 16-byte reads of different element columns on distinct banks */
+    static constexpr int CFS = ((3 * NP + 3) / 4 | 1) * 4;
+    static constexpr int WAVE_FLOATS = 2 * NE * ESTR + NE * CFS;
+};
+
+__host__ __device__ static inline int xchunk_of(int n) { return 64 / (n + 1) - 1; }
+
+/* per-lane constants of the element row b the lane holds */
+template <int N> struct RowCoef {
+    float dz[N + 1];    /* D[b][t]   : z-derivative at row b from row t   */
+    float az[N + 1];    /* Dw[t][b]  : z-divergence into row b from row t */
+    float wb;           /* GLL weight of row b                            */
+};
+
+template <int N>
+__device__ __forceinline__ RowCoef<N> row_coef(int b)
+{
+    RowCoef<N> c;
+#pragma unroll
+    for (int t = 0; t <= N; ++t) {
+        float dz = 0.f, az = 0.f;
+#pragma unroll
+        for (int bb = 0; bb <= N; ++bb) {
+            dz = (b == bb) ? opD<N>(bb, t) : dz;
+            az = (b == bb) ? opDw<N>(t, bb) : az;
+        }
+        c.dz[t] = dz;
+        c.az[t] = az;
+    }
+    float wb = 0.f;
+#pragma unroll
+    for (int bb = 0; bb <= N; ++bb) wb = (b == bb) ? opW<N>(bb) : wb;
+    c.wb = wb;
+    return c;
+}
+
+template <bool UNIFORM>
+__device__ __forceinline__ void node_stress(const float *Cu, const SEMSKS_GMEM float *crec, float e0, float e2, float e3,
+                                            float e4, float e5, float &s0, float &s2, float &s3, float &s4, float &s5)
+{
+    float C[15];
+    if constexpr (UNIFORM) {
+#pragma unroll
+        for (int k = 0; k < 15; ++k) C[k] = Cu[k];
+    } else {
+        const SEMSKS_GMEM float4 *r = reinterpret_cast<const SEMSKS_GMEM float4 *>(crec);
+        const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+        C[0] = r0.y; C[1] = r0.z; C[2] = r0.w; C[3] = r1.x; C[4] = r1.y; C[5] = r1.z; C[6] = r1.w;
+        C[7] = r2.x; C[8] = r2.y; C[9] = r2.z; C[10] = r2.w; C[11] = r3.x; C[12] = r3.y; C[13] = r3.z;
+        C[14] = r3.w;
+    }
+    stress5(C, e0, e2, e3, e4, e5, s0, s2, s3, s4, s5);
+}
+
+/* one element row of K_e u_e (Stiff_Matrix: dUj_di_2D, Stress_ij, F_ik,
+   Grid_Stiff) for the lane's row b: f[a][k] = force on node (a, b).  The
+   element's LDS blocks are laid out [a][k][t] (CB floats per column a), so a
+   lane reads column a of all rows with a few 16-byte loads: Ub holds the
+   rows of u, Qb the column tractions. */
+template <int N, bool UNIFORM>
+__device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)[N + 1][3], int b,
+                                          const RowCoef<N> &rc, float *Ub, float *Qb, const float *Cu,
+                                          const SEMSKS_GMEM float *mat, const SEMSKS_GMEM uint16_t *idrow, bool wr)
 {
     constexpr int NP = N + 1;
+    constexpr int CB = RowLayout<N>::CB;
+    if (wr) {
+#pragma unroll
+        for (int a = 0; a < NP; ++a)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) Ub[a * CB + k * NP + b] = u[a][k];
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+        float col[CB];
+#pragma unroll
+        for (int v = 0; v < CB / 4; ++v) {
+            const float4 t4 = *reinterpret_cast<const float4 *>(Ub + a * CB + 4 * v);
+            col[4 * v] = t4.x; col[4 * v + 1] = t4.y; col[4 * v + 2] = t4.z; col[4 * v + 3] = t4.w;
+        }
+        float gx0 = 0.f, gx1 = 0.f, gx2 = 0.f, gz0 = 0.f, gz1 = 0.f, gz2 = 0.f;
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+            const float dx = opD<N>(a, t);
+            gx0 += dx * u[t][0];
+            gx1 += dx * u[t][1];
+            gx2 += dx * u[t][2];
+            gz0 += rc.dz[t] * col[t];
+            gz1 += rc.dz[t] * col[NP + t];
+            gz2 += rc.dz[t] * col[2 * NP + t];
+        }
+        float s0, s2, s3, s4, s5;
+        const SEMSKS_GMEM float *crec = UNIFORM ? nullptr : mat + (long)idrow[a] * SEMSKS_MAT_STRIDE;
+        node_stress<UNIFORM>(Cu, crec, gx0, gz2, gz1, gx2 + gz0, gx1, s0, s2, s3, s4, s5);
+        const float wa = opW<N>(a);
+        const float p0 = rc.wb * s0, p1 = rc.wb * s5, p2 = rc.wb * s4;   /* row traction    T_i0 */
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const float ax = opDw<N>(a, k);
+            f[k][0] += ax * p0;
+            f[k][1] += ax * p1;
+            f[k][2] += ax * p2;
+        }
+        if (wr) {                                                        /* column traction T_i2 */
+            Qb[a * CB + 0 * NP + b] = wa * s4;
+            Qb[a * CB + 1 * NP + b] = wa * s3;
+            Qb[a * CB + 2 * NP + b] = wa * s2;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+        float col[CB];
+#pragma unroll
+        for (int v = 0; v < CB / 4; ++v) {
+            const float4 t4 = *reinterpret_cast<const float4 *>(Qb + a * CB + 4 * v);
+            col[4 * v] = t4.x; col[4 * v + 1] = t4.y; col[4 * v + 2] = t4.z; col[4 * v + 3] = t4.w;
+        }
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+            f[a][0] += rc.az[t] * col[t];
+            f[a][1] += rc.az[t] * col[NP + t];
+            f[a][2] += rc.az[t] * col[2 * NP + t];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+/* MIXED = false: every element row of the tile (over the wave's element
+   columns) holds one material record, which the host has checked; the
+   record sits in scalar registers.  MIXED = true: material per node. */
+template <int N, bool FORCE_ONLY, bool MIXED>
+__device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows, int lane, float *lds)
+{
+    using RL = RowLayout<N>;
+    constexpr int NP = RL::NP;
     const int nex = D.nex, nez = D.nez, ld = D.ld;
-    const int ex = x0 - 1 + lane;
-    const bool valid = ex >= 0 && ex < nex;
-    const bool own_x = valid && lane >= 1;
+    const int e = lane / NP, b = lane - e * NP;
+    const bool act = lane < RL::NL;
+    const int ex = x0 - 1 + e;
+    const bool valid = act && ex >= 0 && ex < nex;
+    const bool own_x = valid && e >= 1;
     const bool last_x = ex == nex - 1;
-    const bool direct = valid && (lane == 63 || last_x);
+    const bool direct = valid && (e == RL::NE - 1 || last_x);
+    const bool kahan = D.kahan != 0;
+    const int ec = act ? e : 0;
+    float *Ub = lds + ec * RL::ESTR;
+    float *Qb = lds + RL::NE * RL::ESTR + ec * RL::ESTR;
+    float *CF = lds + 2 * RL::NE * RL::ESTR + ec * RL::CFS;
+    const int exl = valid ? ex : 0;
+    const int ex_first = x0 > 0 ? x0 - 1 : 0;
+    const RowCoef<N> rc = row_coef<N>(b);
 
-    const float *__restrict__ U0 = FORCE_ONLY ? D.u[0][0] : D.u[p][0];
-    const float *__restrict__ U1 = FORCE_ONLY ? D.u[0][1] : D.u[p][1];
-    const float *__restrict__ U2 = FORCE_ONLY ? D.u[0][2] : D.u[p][2];
+    const SEMSKS_GMEM float *U0 = gp(FORCE_ONLY ? D.u[0][0] : D.u[p][0]);
+    const SEMSKS_GMEM float *U1 = gp(FORCE_ONLY ? D.u[0][1] : D.u[p][1]);
+    const SEMSKS_GMEM float *U2 = gp(FORCE_ONLY ? D.u[0][2] : D.u[p][2]);
 
-    float u[NP][NP][3];
-    float f[NP][NP][3];
-    int ids[NP][NP];
+    /* carried raw top-row force of the element below, one block per element */
+    if (act && b == N) {
+#pragma unroll
+        for (int i = 0; i < RL::CFS; ++i) CF[i] = 0.f;
+    }
+    __builtin_amdgcn_wave_barrier();
 
     const int ez0 = z0 > 0 ? z0 - 1 : 0;
     const int ez1 = z0 + rows;
     for (int ez = ez0; ez < ez1; ++ez) {
-        const bool first = ez == ez0;
-        /* rows of u: row 0 comes from the previous element's top row, and the
-           force accumulators start from the carried top-row contributions */
-#pragma unroll
-        for (int b = 0; b < NP; ++b) {
-#pragma unroll
-            for (int a = 0; a < NP; ++a)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) f[a][b][i] = (b == 0 && !first) ? f[a][N][i] : 0.f;
-        }
-#pragma unroll
-        for (int b = 0; b < NP; ++b) {
-            if (b == 0 && !first) {
-#pragma unroll
-                for (int a = 0; a < NP; ++a)
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) u[a][0][i] = u[a][N][i];
-                continue;
-            }
-            const long rowoff = (long)(ez * N + b) * ld;
-            float v0[NP], v1[NP], v2[NP];
-            load_cols<N>(U0 + rowoff, ex, valid, direct, v0);
-            load_cols<N>(U1 + rowoff, ex, valid, direct, v1);
-            load_cols<N>(U2 + rowoff, ex, valid, direct, v2);
-#pragma unroll
-            for (int a = 0; a < NP; ++a) { u[a][b][0] = v0[a]; u[a][b][1] = v1[a]; u[a][b][2] = v2[a]; }
-        }
-        /* material: one record for the whole wave when every element in it
-           is uniform with the same record, else per node */
-        const int em = valid ? (int)D.emat[(long)ez * nex + ex] : -1;
-        int ref;
+        const int iz = ez * N + b;
+        const bool top = ez == nez - 1;
+        const bool own = own_x && ez >= z0 && (b < N || top);
+        const long rowoff = (long)iz * ld + (long)exl * N;
+        /* this lane's row of u(t) */
+        float u[NP][3];
         {
-            const unsigned long long vm = __ballot(valid);
-            const int src = vm ? __ffsll((long long)vm) - 1 : 0;
-            ref = __shfl(em, src, 64);
+            const SEMSKS_GMEM float *Uk[3] = {U0, U1, U2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (valid) {
+                    if constexpr (N == 4) {
+                        const float4 t = *reinterpret_cast<const SEMSKS_GMEM float4 *>(Uk[k] + rowoff);
+                        u[0][k] = t.x; u[1][k] = t.y; u[2][k] = t.z; u[3][k] = t.w;
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < N; ++a) u[a][k] = Uk[k][rowoff + a];
+                    }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < N; ++a) u[a][k] = 0.f;
+                }
+                float nb = __shfl(u[0][k], lane + NP, 64);
+                if (direct) nb = Uk[k][rowoff + N];
+                u[N][k] = nb;
+            }
         }
-        const bool uniform = __all(!valid || em == ref) && ref != 0xFFFF && ref >= 0;
-        if (uniform) {
-            const int rid = __builtin_amdgcn_readfirstlane(ref);
-            const float *r = D.mat + (long)rid * SEMSKS_MAT_STRIDE;
+        /* the update state of the row: fetched now, consumed after the force */
+        const bool vec = N == 4 && !FORCE_ONLY && own && b < N && iz != 0 && ex != 0;
+        const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
+        int rid = 0;
+        if constexpr (!MIXED) rid = __builtin_amdgcn_readfirstlane((int)gp(D.emat)[(long)ez * nex + ex_first]);
+        float4 pd[3] = {};
+        uint4 pc = make_uint4(0u, 0u, 0u, 0u);
+        float m[4] = {0.f, 0.f, 0.f, 0.f};     /* dt^2/M of the row's four owned nodes */
+        if constexpr (N == 4 && !FORCE_ONLY) {
+            if (vec) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pd[k] = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.del[k]) + rowoff);
+                if (kahan) pc = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(gp(D.cmp) + rowoff);
+                /* mass class: cz = b (0 = row shared with the element below) */
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int mid = MIXED ? (int)idrow[a] : rid;
+                    m[a] = gp(D.mclass)[(long)mid * 25 + b * 5 + a];
+                }
+            }
+        }
+        float f[NP][3];
+#pragma unroll
+        for (int a = 0; a < NP; ++a)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) f[a][k] = 0.f;
+        if constexpr (!MIXED) {
+            const SEMSKS_GMEM float *r = gp(D.mat) + (long)rid * SEMSKS_MAT_STRIDE;
             float C[15];
 #pragma unroll
             for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
-            element_force<N, true>(u, f, C, D.mat, ids);
+            row_force<N, true>(u, f, b, rc, Ub, Qb, C, gp(D.mat), idrow, act);
         } else {
-#pragma unroll
-            for (int b = 0; b < NP; ++b) {
-                int v[NP];
-                load_ids<N>(D.mat_id + (long)(ez * N + b) * ld, ex, valid, direct, v);
-#pragma unroll
-                for (int a = 0; a < NP; ++a) ids[a][b] = v[a];
-            }
-            element_force<N, false>(u, f, nullptr, D.mat, ids);
+            row_force<N, false>(u, f, b, rc, Ub, Qb, nullptr, gp(D.mat), idrow, act);
         }
         if (!valid) {
 #pragma unroll
             for (int a = 0; a < NP; ++a)
 #pragma unroll
-                for (int b = 0; b < NP; ++b)
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) f[a][b][i] = 0.f;
+                for (int k = 0; k < 3; ++k) f[a][k] = 0.f;
         }
-        const bool top = ez == nez - 1;
-        /* left column gets the right column of the lane to the left */
+        /* z assembly: the bottom row adds the raw top row of the element
+           below, then the raw top row is handed up through LDS */
+        if (b == 0) {
+            float cf[RL::CFS];
 #pragma unroll
-        for (int b = 0; b < NP; ++b) {
-            if (b == N && !top) continue;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                float s = __shfl_up(f[N][b][i], 1, 64);
-                if (lane > 0) f[0][b][i] += s;
+            for (int v = 0; v < (3 * NP + 3) / 4; ++v) {
+                const float4 t4 = *reinterpret_cast<const float4 *>(CF + 4 * v);
+                cf[4 * v] = t4.x; cf[4 * v + 1] = t4.y; cf[4 * v + 2] = t4.z; cf[4 * v + 3] = t4.w;
             }
-        }
-        if (own_x && ez >= z0) {
-            const int a1 = last_x ? NP : N;
 #pragma unroll
-            for (int b = 0; b < NP; ++b) {
-                if (b == N && !top) continue;
-                const int iz = ez * N + b;
-                update_row<N, FORCE_ONLY>(D, p, (long)iz * ld + ex * N, ex * N, iz, a1, f, u, b);
+            for (int a = 0; a < NP; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) f[a][k] += cf[a * 3 + k];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (act && b == N) {
+#pragma unroll
+            for (int a = 0; a < NP; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) CF[a * 3 + k] = f[a][k];
+        }
+        __builtin_amdgcn_wave_barrier();
+        /* x assembly: the left column adds the right column of the element
+           to the left */
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float s = __shfl(f[N][k], lane - NP, 64);
+            if (e > 0) f[0][k] += s;
+        }
+        if (!own) continue;
+        const int ixb = ex * N;
+        const long q0 = (long)iz * ld + ixb;
+        const int a1 = last_x ? NP : N;
+        if constexpr (FORCE_ONLY) {
+#pragma unroll
+            for (int a = 0; a < NP; ++a)
+                if (a < a1)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) gp(D.fout[k])[q0 + a] = f[a][k];
+            continue;
+        } else {
+            if constexpr (N == 4) {
+                if (vec) {
+                    float4 un[3];
+                    uint4 cw = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        float d[4] = {pd[k].x, pd[k].y, pd[k].z, pd[k].w};
+                        float v[4];
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) {
+                            d[a] = d[a] - m[a] * f[a][k];
+                            v[a] = u[a][k];
+                            if (kahan) {
+                                const uint32_t wc = a == 0 ? pc.x : a == 1 ? pc.y : a == 2 ? pc.z : pc.w;
+                                float c = c_unpack(wc, k, v[a]);
+                                kahan_add(v[a], c, d[a]);
+                                const uint32_t bits = c_pack(c, v[a]) << (8 * k);
+                                if (a == 0) cw.x |= bits; else if (a == 1) cw.y |= bits; else if (a == 2) cw.z |= bits; else cw.w |= bits;
+                            } else {
+                                v[a] = v[a] + d[a];
+                            }
+                        }
+                        pd[k] = make_float4(d[0], d[1], d[2], d[3]);
+                        un[k] = make_float4(v[0], v[1], v[2], v[3]);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.del[k]) + q0) = pd[k];
+                        *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[p ^ 1][k]) + q0) = un[k];
+                    }
+                    if (kahan) *reinterpret_cast<SEMSKS_GMEM uint4 *>(gp(D.cmp) + q0) = cw;
+                    if (last_x) {
+                        float fa[3] = {f[N][0], f[N][1], f[N][2]};
+                        float ua[3] = {u[N][0], u[N][1], u[N][2]};
+                        update_one<N>(D, p, q0 + N, ixb + N, iz, MIXED ? (int)idrow[N] : rid, fa, ua);
+                    }
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < NP; ++a) {
+                if (a >= a1) continue;
+                float fa[3] = {f[a][0], f[a][1], f[a][2]};
+                float ua[3] = {u[a][0], u[a][1], u[a][2]};
+                update_one<N>(D, p, q0 + a, ixb + a, iz, MIXED ? (int)idrow[a] : rid, fa, ua);
             }
         }
     }
 }
 
-/* persistent grid: each 64-lane wave takes tiles (slot, x0, z0, rows) from the
-   work list until it is exhausted */
+/* persistent grid: each 64-lane wave takes tiles (slot, x0, z0, rows | mixed
+   << 16) from the work list until it is exhausted; a mixed tile takes the
+   per-node material path */
 template <int N, bool FORCE_ONLY>
-__global__ __launch_bounds__(256) void step_kernel(const InstDesc *__restrict__ insts, const int *__restrict__ levels,
-                                                   const int4 *__restrict__ tiles, const int *__restrict__ ntiles)
+__global__ __launch_bounds__(256, 3) void step_kernel(const InstDesc *__restrict__ insts, const int *__restrict__ levels,
+                                                      const int4 *__restrict__ tiles, const int *__restrict__ ntiles)
 {
+    __shared__ float lds_blocks[4][RowLayout<N>::WAVE_FLOATS];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -765,9 +941,12 @@ __global__ __launch_bounds__(256) void step_kernel(const InstDesc *__restrict__ 
             if (it >= D.nst - 1) continue;
             p = it & 1;
         }
-        process_tile<N, FORCE_ONLY>(D, p, tile.y, tile.z, tile.w, lane);
+        float *lds = lds_blocks[threadIdx.x >> 6];
+        if (tile.w >> 16) process_tile<N, FORCE_ONLY, true>(D, p, tile.y, tile.z, tile.w & 0xFFFF, lane, lds);
+        else process_tile<N, FORCE_ONLY, false>(D, p, tile.y, tile.z, tile.w, lane, lds);
     }
 }
+
 
 /* ========================================================================= */
 /* closed-form plane wave records (stand-in for the GRT producer)            */
@@ -822,7 +1001,41 @@ struct Slot {
     int nx = 0, nz = 0, ld = 0, nt_sem = 0, nlev = 0;
     size_t bytes = 0;
     std::vector<void *> allocs;       /* owned device allocations */
+    std::vector<uint16_t> emat;       /* host copy of the element-uniform record ids (owner) */
 };
+
+/* A tile may run the single-record kernel when each of its element rows
+   (including the halo row below and the halo column to the left) holds one
+   material record over the tile's element columns. */
+bool tile_single_record(const Slot &o, int n, int x0, int z0, int rows)
+{
+    const int nex = o.meta.nex, nez = o.meta.nez;
+    const int ex0 = x0 > 0 ? x0 - 1 : 0;
+    const int ex1 = std::min(nex, x0 - 1 + 64 / (n + 1));
+    const int ez0 = z0 > 0 ? z0 - 1 : 0;
+    const int ez1 = std::min(nez, z0 + rows);
+    for (int ez = ez0; ez < ez1; ++ez) {
+        const uint16_t r = o.emat[(size_t)ez * nex + ex0];
+        if (r == 0xFFFF) return false;
+        for (int ex = ex0 + 1; ex < ex1; ++ex)
+            if (o.emat[(size_t)ez * nex + ex] != r) return false;
+    }
+    return true;
+}
+
+/* tiles of one slot's mesh as (slot, x0, z0, rows | mixed << 16), the mixed
+   (per-node material) tiles first since they take longest */
+void build_tiles(const Slot &o, int slot, int tz, std::vector<int4> &t)
+{
+    const int n = o.meta.hpn;
+    for (int mixed = 1; mixed >= 0; --mixed)
+        for (int z0 = 0; z0 < o.meta.nez; z0 += tz)
+            for (int x0 = 0; x0 < o.meta.nex; x0 += xchunk_of(n)) {
+                const int rows = std::min(tz, o.meta.nez - z0);
+                if (tile_single_record(o, n, x0, z0, rows) == (mixed != 0)) continue;
+                t.push_back(make_int4(slot, x0, z0, rows | (mixed << 16)));
+            }
+}
 
 template <class T>
 hipError_t dalloc(Slot &s, T **p, size_t n)
@@ -878,27 +1091,30 @@ static semsks_status upload_tables(semsks_engine *e)
         h[i].active = e->slots[i].used ? 1 : 0;
     }
     HIPCHK(hipMemcpyAsync(e->d_desc, h.data(), h.size() * sizeof(InstDesc), hipMemcpyHostToDevice, e->stream));
-    /* tiles, grouped by hpn, slots interleaved per spatial tile so that
-       concurrent waves touch the same region of a shared mesh */
+    /* tiles, grouped by hpn, mixed (per-node material) tiles first, slots
+       interleaved per spatial tile so that concurrent waves touch the same
+       region of a shared mesh */
     std::vector<int4> t;
     int cnt[SEMSKS_MAXNP] = {0};
     e->hpn_mask = 0;
+    const int tzg = e->opts.tile_z > 0 ? e->opts.tile_z : 8;
     for (int n = 3; n <= 6; ++n) {
         e->tile_off[n] = (int)t.size();
         int maxx = 0, maxz = 0;
         for (auto &s : e->slots)
             if (s.used && s.meta.hpn == n) { maxx = std::max(maxx, s.meta.nex); maxz = std::max(maxz, s.meta.nez); }
-        const int tzg = e->opts.tile_z > 0 ? e->opts.tile_z : 8;
-        for (int z0 = 0; z0 < maxz; z0 += tzg) {
-            for (int x0 = 0; x0 < maxx; x0 += SEMSKS_XCHUNK) {
-                for (size_t k = 0; k < e->slots.size(); ++k) {
-                    const Slot &s = e->slots[k];
-                    if (!s.used || s.meta.hpn != n) continue;
-                    if (z0 >= s.meta.nez || x0 >= s.meta.nex) continue;
-                    t.push_back(make_int4((int)k, x0, z0, std::min(tzg, s.meta.nez - z0)));
-                }
-            }
-        }
+        for (int mixed = 1; mixed >= 0; --mixed)
+            for (int z0 = 0; z0 < maxz; z0 += tzg)
+                for (int x0 = 0; x0 < maxx; x0 += xchunk_of(n))
+                    for (size_t k = 0; k < e->slots.size(); ++k) {
+                        const Slot &s = e->slots[k];
+                        if (!s.used || s.meta.hpn != n) continue;
+                        if (z0 >= s.meta.nez || x0 >= s.meta.nex) continue;
+                        const Slot &o = s.share >= 0 ? e->slots[s.share] : s;
+                        const int rows = std::min(tzg, s.meta.nez - z0);
+                        if (tile_single_record(o, n, x0, z0, rows) == (mixed != 0)) continue;
+                        t.push_back(make_int4((int)k, x0, z0, rows | (mixed << 16)));
+                    }
         cnt[n] = (int)t.size() - e->tile_off[n];
         if (cnt[n]) e->hpn_mask |= 1u << n;
     }
@@ -915,10 +1131,42 @@ static semsks_status upload_tables(semsks_engine *e)
     return SEMSKS_OK;
 }
 
-static int persistent_blocks(const semsks_engine *e)
+
+template <bool FORCE_ONLY> const void *step_fn(int n)
 {
-    int wpc = e->opts.waves_per_cu > 0 ? e->opts.waves_per_cu : 8;
-    return std::max(1, e->n_cu * wpc / 4);
+    switch (n) {
+    case 3: return reinterpret_cast<const void *>(&step_kernel<3, FORCE_ONLY>);
+    case 4: return reinterpret_cast<const void *>(&step_kernel<4, FORCE_ONLY>);
+    case 5: return reinterpret_cast<const void *>(&step_kernel<5, FORCE_ONLY>);
+    default: return reinterpret_cast<const void *>(&step_kernel<6, FORCE_ONLY>);
+    }
+}
+
+/* persistent grid: as many 256-thread blocks as the kernel keeps resident on
+   every CU (or waves_per_cu / 4 per CU when the option is set) */
+template <bool FORCE_ONLY>
+static int persistent_blocks(const semsks_engine *e, int n)
+{
+    if (e->opts.waves_per_cu > 0) return std::max(1, e->n_cu * e->opts.waves_per_cu / 4);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_fn<FORCE_ONLY>(n), 256, 0) != hipSuccess || per_cu < 1)
+        per_cu = 2;
+    return e->n_cu * per_cu;
+}
+
+/* one launch of the element-force kernel for the tiles of polynomial degree n */
+template <bool FORCE_ONLY>
+static void launch_tiles(const semsks_engine *e, int n, const InstDesc *desc, const int *levels, const int4 *tl,
+                         const int *nt)
+{
+    const int nb = persistent_blocks<FORCE_ONLY>(e, n);
+    hipStream_t st = e->stream;
+    switch (n) {
+    case 3: hipLaunchKernelGGL((step_kernel<3, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt); break;
+    case 4: hipLaunchKernelGGL((step_kernel<4, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt); break;
+    case 5: hipLaunchKernelGGL((step_kernel<5, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt); break;
+    case 6: hipLaunchKernelGGL((step_kernel<6, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt); break;
+    }
 }
 
 static semsks_status launch_step(semsks_engine *e, hipEvent_t ta, hipEvent_t tb)
@@ -926,18 +1174,9 @@ static semsks_status launch_step(semsks_engine *e, hipEvent_t ta, hipEvent_t tb)
     const int nslots = (int)e->slots.size();
     hipLaunchKernelGGL(inject_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
     if (ta) HIPCHK(hipEventRecord(ta, e->stream));
-    const int nb = persistent_blocks(e);
-    for (int n = 3; n <= 6; ++n) {
-        if (!(e->hpn_mask & (1u << n))) continue;
-        const int4 *tl = e->d_tiles + e->tile_off[n];
-        const int *nt = e->d_ntiles + n;
-        switch (n) {
-        case 3: hipLaunchKernelGGL((step_kernel<3, false>), dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels, tl, nt); break;
-        case 4: hipLaunchKernelGGL((step_kernel<4, false>), dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels, tl, nt); break;
-        case 5: hipLaunchKernelGGL((step_kernel<5, false>), dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels, tl, nt); break;
-        case 6: hipLaunchKernelGGL((step_kernel<6, false>), dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels, tl, nt); break;
-        }
-    }
+    for (int n = 3; n <= 6; ++n)
+        if (e->hpn_mask & (1u << n))
+            launch_tiles<false>(e, n, e->d_desc, e->d_levels, e->d_tiles + e->tile_off[n], e->d_ntiles + n);
     if (tb) HIPCHK(hipEventRecord(tb, e->stream));
     hipLaunchKernelGGL(record_kernel, dim3(nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
     HIPCHK(hipGetLastError());
@@ -1034,8 +1273,9 @@ size_t semsks_footprint(const semsks_problem *p, int shared)
 {
     const size_t nodes = (size_t)ld_of(p) * nz_of(p);
     const int kahan = !(p->flags & SEMSKS_F_NO_KAHAN);
-    size_t b = nodes * 4 * (6 + 3 + (kahan ? 3 : 0));                  /* u x2, delta, comp */
-    if (!shared) b += nodes * (2 + 4) + (size_t)p->nex * p->nez * 2 + (size_t)p->n_mat * 64 +
+    size_t b = nodes * 4 * (6 + 3 + (kahan ? 1 : 0));                  /* u x2, delta, packed comp */
+    if (!shared) b += nodes * 2 + (size_t)p->nex * p->nez * 2 +
+                      (size_t)p->n_mat * (64 + 4 * (p->hpn + 1) * (p->hpn + 1)) +
                       (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24;
     const int mt = 2 * (nx_of(p) + nz_of(p)), sht = nx_of(p) + nz_of(p);
     int nlev = std::min(std::max(nt_sem_of(p), 1), std::max(p->nst, 1));
@@ -1122,30 +1362,30 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
 #define ACHK(x) do { if ((x) != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "device allocation failed"); goto fail; } } while (0)
     {
         float *st_all = nullptr;
-        const int nplanes = 6 + 3 + (d.kahan ? 3 : 0);
+        const int nplanes = 6 + 3 + (d.kahan ? 1 : 0);
         ACHK(dalloc(s, &st_all, nodes * nplanes));
         ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->stream));
         for (int b = 0; b < 2; ++b) for (int c = 0; c < 3; ++c) d.u[b][c] = st_all + (b * 3 + c) * nodes;
         for (int c = 0; c < 3; ++c) d.del[c] = st_all + (6 + c) * nodes;
-        for (int c = 0; c < 3; ++c) d.cmp[c] = d.kahan ? st_all + (9 + c) * nodes : nullptr;
+        d.cmp = d.kahan ? reinterpret_cast<uint32_t *>(st_all + 9 * nodes) : nullptr;
     }
     if (share >= 0) {
         const InstDesc &o = e->slots[share].d;
-        d.mat_id = o.mat_id; d.emat = o.emat; d.mat = o.mat; d.mdt2 = o.mdt2; d.edge_a = o.edge_a; d.edge_b = o.edge_b;
+        d.mat_id = o.mat_id; d.emat = o.emat; d.mat = o.mat; d.mclass = o.mclass; d.edge_a = o.edge_a; d.edge_b = o.edge_b;
         memcpy(d.Dg, o.Dg, sizeof d.Dg); memcpy(d.Dw, o.Dw, sizeof d.Dw); memcpy(d.cw, o.cw, sizeof d.cw);
         e->slots[share].refs++;
     } else {
         uint16_t *mid = nullptr, *emat = nullptr;
-        float *mat = nullptr, *mdt2 = nullptr, *ea = nullptr, *eb = nullptr;
+        float *mat = nullptr, *mcl = nullptr, *ea = nullptr, *eb = nullptr;
+        const int ncls = (p->hpn + 1) * (p->hpn + 1);
         ACHK(dalloc(s, &mid, nodes));
         ACHK(dalloc(s, &emat, (size_t)p->nex * p->nez));
         ACHK(dalloc(s, &mat, (size_t)p->n_mat * SEMSKS_MAT_STRIDE));
-        ACHK(dalloc(s, &mdt2, nodes));
+        ACHK(dalloc(s, &mcl, (size_t)p->n_mat * ncls));
         const int ne = semsks_n_edge(s.nx, s.nz, p->hpn);
         ACHK(dalloc(s, &ea, (size_t)ne * 3));
         ACHK(dalloc(s, &eb, (size_t)ne * 3));
         ACHK(hipMemsetAsync(mid, 0, nodes * 2, e->stream));
-        ACHK(hipMemsetAsync(mdt2, 0, nodes * 4, e->stream));
         ACHK(hipMemcpy2DAsync(mid, s.ld * 2, p->mat_id, s.nx * 2, s.nx * 2, s.nz, hipMemcpyHostToDevice, e->stream));
         ACHK(hipMemcpyAsync(mat, p->mat_table, (size_t)p->n_mat * SEMSKS_MAT_STRIDE * 4, hipMemcpyHostToDevice, e->stream));
         {   /* element-uniform record ids */
@@ -1161,8 +1401,9 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
                 }
             ACHK(hipMemcpyAsync(emat, em.data(), em.size() * 2, hipMemcpyHostToDevice, e->stream));
             ACHK(hipStreamSynchronize(e->stream));
+            s.emat = std::move(em);
         }
-        d.mat_id = mid; d.emat = emat; d.mat = mat; d.mdt2 = mdt2; d.edge_a = ea; d.edge_b = eb;
+        d.mat_id = mid; d.emat = emat; d.mat = mat; d.mclass = mcl; d.edge_a = ea; d.edge_b = eb;
         /* FP64 setup on the device */
         {
             InstDesc *dd = nullptr;
@@ -1178,6 +1419,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
             hipLaunchKernelGGL(jacobian_kernel, dim3(256), dim3(256), 0, e->stream, p->nex, p->nez, p->hpn, (double)p->eh, dev);
             const double eh = p->eh, je = (double)((double)eh * eh / 4.0), jb = eh / 2.0;
             hipLaunchKernelGGL(mass_kernel, dim3(1024), dim3(256), 0, e->stream, d, w64, je, p->sdt, mass);
+            hipLaunchKernelGGL(mclass_kernel, dim3(64), dim3(256), 0, e->stream, mat, p->n_mat, p->hpn, w64, je, p->sdt, mcl);
             hipLaunchKernelGGL(absorb_kernel, dim3(64), dim3(256), 0, e->stream, d, w64, jb, p->sdt, p->absorb, mass,
                                (double *)nullptr, ea, eb);
             InstDesc back;
@@ -1408,23 +1650,15 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     int4 *tl = nullptr;
     int *nt = nullptr;
     std::vector<int4> t;
-    const int tz = tile_rows(e, s.meta.nez);
-    for (int z0 = 0; z0 < s.meta.nez; z0 += tz)
-        for (int x0 = 0; x0 < s.meta.nex; x0 += SEMSKS_XCHUNK) t.push_back(make_int4(0, x0, z0, std::min(tz, s.meta.nez - z0)));
-    int cnt = (int)t.size();
+    build_tiles(s.share >= 0 ? e->slots[s.share] : s, 0, tile_rows(e, s.meta.nez), t);
+    const int cnt = (int)t.size();
     HIPCHK(hipMalloc(&dd, sizeof d));
     HIPCHK(hipMalloc(&tl, t.size() * sizeof(int4)));
-    HIPCHK(hipMalloc(&nt, sizeof(int)));
+    HIPCHK(hipMalloc(&nt, sizeof cnt));
     HIPCHK(hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(tl, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(nt, &cnt, sizeof cnt, hipMemcpyHostToDevice, e->stream));
-    const int nb = persistent_blocks(e);
-    switch (s.meta.hpn) {
-    case 3: hipLaunchKernelGGL((step_kernel<3, true>), dim3(nb), dim3(256), 0, e->stream, dd, e->d_levels, tl, nt); break;
-    case 4: hipLaunchKernelGGL((step_kernel<4, true>), dim3(nb), dim3(256), 0, e->stream, dd, e->d_levels, tl, nt); break;
-    case 5: hipLaunchKernelGGL((step_kernel<5, true>), dim3(nb), dim3(256), 0, e->stream, dd, e->d_levels, tl, nt); break;
-    case 6: hipLaunchKernelGGL((step_kernel<6, true>), dim3(nb), dim3(256), 0, e->stream, dd, e->d_levels, tl, nt); break;
-    }
+    launch_tiles<true>(e, s.meta.hpn, dd, e->d_levels, tl, nt);
     HIPCHK(hipGetLastError());
     for (int c = 0; c < 3; ++c)
         HIPCHK(hipMemcpy2DAsync(f + (size_t)c * s.nx * s.nz, s.nx * 4, d.fout[c], s.ld * 4, s.nx * 4, s.nz,
@@ -1451,17 +1685,17 @@ semsks_status semsks_get_setup(semsks_engine *e, int k, double *mass, float *mdt
     HIPCHK(hipMalloc(&eb, ne * 3 * sizeof(float)));
     HIPCHK(hipMemsetAsync(dab, 0, N * 3 * sizeof(double), e->stream));
     InstDesc d = o.d, *dd = nullptr;
-    d.mdt2 = md;
     HIPCHK(hipMalloc(&dd, sizeof d));
     HIPCHK(hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, e->stream));
     const double eh = s.meta.eh;
     hipLaunchKernelGGL(setup_tables_kernel, dim3(1), dim3(64), 0, e->stream, dd, w64, eh);
     hipLaunchKernelGGL(mass_kernel, dim3(1024), dim3(256), 0, e->stream, d, w64, (double)(eh * eh / 4.0), s.meta.sdt, dm);
     hipLaunchKernelGGL(absorb_kernel, dim3(64), dim3(256), 0, e->stream, d, w64, eh / 2.0, s.meta.sdt, s.meta.absorb, dm, dab, ea, eb);
+    hipLaunchKernelGGL(expand_mdt2_kernel, dim3(256), dim3(256), 0, e->stream, d, md);
     HIPCHK(hipGetLastError());
     if (mass) HIPCHK(hipMemcpyAsync(mass, dm, N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     if (ab) HIPCHK(hipMemcpyAsync(ab, dab, N * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-    if (mdt2) HIPCHK(hipMemcpy2DAsync(mdt2, s.nx * 4, md, s.ld * 4, s.nx * 4, s.nz, hipMemcpyDeviceToHost, e->stream));
+    if (mdt2) HIPCHK(hipMemcpyAsync(mdt2, md, N * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     hipFree(w64); hipFree(dm); hipFree(dab); hipFree(md); hipFree(ea); hipFree(eb); hipFree(dd);
     return SEMSKS_OK;

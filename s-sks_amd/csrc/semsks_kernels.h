@@ -6,12 +6,15 @@
  *             row stride ld floats (ld >= nx+3, multiple of 64) so that an
  *             element's first hpn columns are one aligned vector load
  *   del[3]    delta = u(t) - u(t-dt), in place
- *   cmp[3]    Kahan compensation of u, in place (absent without Kahan)
+ *   cmp       Kahan compensation of u, in place: one 32-bit word per node
+ *             holding three signed bytes in units of ulp(u)/256 (absent
+ *             without Kahan)
  *   mat_id    16-bit material record id per node (same row stride)
  *   emat      16-bit record id per element when all its nodes share one
  *             record, 0xFFFF otherwise
  *   mat       records {rho, 15 constants}, 16 floats each
- *   mdt2      dt^2/M per node (FP32, from the FP64 lumped mass)
+ *   mclass    dt^2/M per material record and node position class
+ *             [n_mat][hpn+1][hpn+1] (FP32 from the FP64 lumped mass)
  *   edge_a/b  (M-A)/(M+A) and dt^2/(M+A) per absorbing-edge node, 3 comps
  *   G, Gv     interpolated, guarded boundary vectors per time level
  *   traces    [nst][3][ntrace] receiver samples (u, -w, v)
@@ -22,19 +25,18 @@
 #include <stdint.h>
 
 #define SEMSKS_MAXNP 7
-#define SEMSKS_XCHUNK 63   /* owned element columns per 64-lane wave tile */
 
 struct InstDesc {
     int nex, nez, nx, nz, ld;
     int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active;
     float *u[2][3];
     float *del[3];
-    float *cmp[3];
+    uint32_t *cmp;
     float *fout[3];                 /* force-only mode output, else null */
     const uint16_t *mat_id;
     const uint16_t *emat;
     const float *mat;
-    const float *mdt2;
+    const float *mclass;
     const float *edge_a, *edge_b;   /* [n_edge][3] */
     const float *G, *Gv;            /* [nt_sem][2(nx+nz)], [nt_sem][nx+nz] */
     const int *rec;                 /* node offsets iz*ld+ix */
@@ -48,6 +50,22 @@ struct InstDesc {
     float cw[SEMSKS_MAXNP];
     int pad[5];
 };
+
+#if defined(__HIPCC__)
+/* Pointers taken out of an InstDesc are generic to the compiler; casting
+   them to the global address space makes the hot loops use global_load /
+   global_store instead of FLAT memory operations (which also occupy the
+   LDS counter and so serialise with cross-lane shuffles). */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SEMSKS_GMEM __attribute__((address_space(1)))
+#else
+#define SEMSKS_GMEM
+#endif
+template <class T>
+__device__ __forceinline__ SEMSKS_GMEM T *gp(T *p) { return (SEMSKS_GMEM T *)p; }
+template <class T>
+__device__ __forceinline__ const SEMSKS_GMEM T *gp(const T *p) { return (const SEMSKS_GMEM T *)p; }
+#endif
 
 /* edge-node numbering: bottom row iz=0 (nx) | left column iz>=1 (nz-1) |
    right column iz>=1 (nz-1) | top row iz=nz-1, ix=1..3*hpn (3*hpn) */
