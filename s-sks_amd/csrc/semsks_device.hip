@@ -717,6 +717,20 @@ __device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)
     __builtin_amdgcn_wave_barrier();
 }
 
+/* What a lane fetches from global memory for one element row: its row of
+   u (columns 0..N-1, plus column N on lanes whose right neighbour lane does
+   not hold it), and, for the update, the row's delta, packed compensation
+   and dt^2/M.  The next row's fetch is issued before the current row's
+   stores, so waiting for it never waits for those stores. */
+template <int N> struct RowFetch {
+    float u[N][3];
+    float un[3];
+    float4 pd[3];
+    uint4 pc;
+    float m[4];
+    int rid;           /* single-record tiles: the row's material record */
+};
+
 /* MIXED = false: every element row of the tile (over the wave's element
    columns) holds one material record, which the host has checked; the
    record sits in scalar registers.  MIXED = true: material per node. */
@@ -746,6 +760,52 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     const SEMSKS_GMEM float *U1 = gp(FORCE_ONLY ? D.u[0][1] : D.u[p][1]);
     const SEMSKS_GMEM float *U2 = gp(FORCE_ONLY ? D.u[0][2] : D.u[p][2]);
 
+    const int ez0 = z0 > 0 ? z0 - 1 : 0;
+    const int ez1 = z0 + rows;
+
+    auto fetch = [&](int ez, RowFetch<N> &r) {
+        const int iz = ez * N + b;
+        const long rowoff = (long)iz * ld + (long)exl * N;
+        const SEMSKS_GMEM float *Uk[3] = {U0, U1, U2};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (valid) {
+                if constexpr (N == 4) {
+                    const float4 t = *reinterpret_cast<const SEMSKS_GMEM float4 *>(Uk[k] + rowoff);
+                    r.u[0][k] = t.x; r.u[1][k] = t.y; r.u[2][k] = t.z; r.u[3][k] = t.w;
+                } else {
+#pragma unroll
+                    for (int a = 0; a < N; ++a) r.u[a][k] = Uk[k][rowoff + a];
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < N; ++a) r.u[a][k] = 0.f;
+            }
+            r.un[k] = direct ? Uk[k][rowoff + N] : 0.f;
+        }
+        r.rid = 0;
+        if constexpr (!MIXED) r.rid = __builtin_amdgcn_readfirstlane((int)gp(D.emat)[(long)ez * nex + ex_first]);
+        if constexpr (N == 4 && !FORCE_ONLY) {
+            const bool own = own_x && ez >= z0 && b < N;
+            const bool vec = own && iz != 0 && ex != 0;
+            r.pd[0] = r.pd[1] = r.pd[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+            r.pc = make_uint4(0u, 0u, 0u, 0u);
+            r.m[0] = r.m[1] = r.m[2] = r.m[3] = 0.f;
+            if (vec) {
+                const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) r.pd[k] = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.del[k]) + rowoff);
+                if (kahan) r.pc = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(gp(D.cmp) + rowoff);
+                /* mass class: cz = b (0 = row shared with the element below) */
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int mid = MIXED ? (int)idrow[a] : r.rid;
+                    r.m[a] = gp(D.mclass)[(long)mid * 25 + b * 5 + a];
+                }
+            }
+        }
+    };
+
     /* carried raw top-row force of the element below, one block per element */
     if (act && b == N) {
 #pragma unroll
@@ -753,56 +813,29 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     }
     __builtin_amdgcn_wave_barrier();
 
-    const int ez0 = z0 > 0 ? z0 - 1 : 0;
-    const int ez1 = z0 + rows;
+    /* single-record tiles fetch one row ahead; the mixed path, which holds
+       per-node material records in registers, fetches at the top of a row */
+    constexpr bool AHEAD = !MIXED;
+    RowFetch<N> cur;
+    fetch(ez0, cur);
     for (int ez = ez0; ez < ez1; ++ez) {
+        if constexpr (!AHEAD) {
+            if (ez > ez0) fetch(ez, cur);
+        }
         const int iz = ez * N + b;
         const bool top = ez == nez - 1;
         const bool own = own_x && ez >= z0 && (b < N || top);
         const long rowoff = (long)iz * ld + (long)exl * N;
-        /* this lane's row of u(t) */
-        float u[NP][3];
-        {
-            const SEMSKS_GMEM float *Uk[3] = {U0, U1, U2};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (valid) {
-                    if constexpr (N == 4) {
-                        const float4 t = *reinterpret_cast<const SEMSKS_GMEM float4 *>(Uk[k] + rowoff);
-                        u[0][k] = t.x; u[1][k] = t.y; u[2][k] = t.z; u[3][k] = t.w;
-                    } else {
-#pragma unroll
-                        for (int a = 0; a < N; ++a) u[a][k] = Uk[k][rowoff + a];
-                    }
-                } else {
-#pragma unroll
-                    for (int a = 0; a < N; ++a) u[a][k] = 0.f;
-                }
-                float nb = __shfl(u[0][k], lane + NP, 64);
-                if (direct) nb = Uk[k][rowoff + N];
-                u[N][k] = nb;
-            }
-        }
-        /* the update state of the row: fetched now, consumed after the force */
-        const bool vec = N == 4 && !FORCE_ONLY && own && b < N && iz != 0 && ex != 0;
         const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
-        int rid = 0;
-        if constexpr (!MIXED) rid = __builtin_amdgcn_readfirstlane((int)gp(D.emat)[(long)ez * nex + ex_first]);
-        float4 pd[3] = {};
-        uint4 pc = make_uint4(0u, 0u, 0u, 0u);
-        float m[4] = {0.f, 0.f, 0.f, 0.f};     /* dt^2/M of the row's four owned nodes */
-        if constexpr (N == 4 && !FORCE_ONLY) {
-            if (vec) {
+        const int rid = cur.rid;
+        /* this lane's row of u(t); column N from the right neighbour lane */
+        float u[NP][3];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) pd[k] = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.del[k]) + rowoff);
-                if (kahan) pc = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(gp(D.cmp) + rowoff);
-                /* mass class: cz = b (0 = row shared with the element below) */
+        for (int k = 0; k < 3; ++k) {
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int mid = MIXED ? (int)idrow[a] : rid;
-                    m[a] = gp(D.mclass)[(long)mid * 25 + b * 5 + a];
-                }
-            }
+            for (int a = 0; a < N; ++a) u[a][k] = cur.u[a][k];
+            const float nb = __shfl(cur.u[0][k], lane + NP, 64);
+            u[N][k] = direct ? cur.un[k] : nb;
         }
         float f[NP][3];
 #pragma unroll
@@ -853,65 +886,70 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
             const float s = __shfl(f[N][k], lane - NP, 64);
             if (e > 0) f[0][k] += s;
         }
-        if (!own) continue;
-        const int ixb = ex * N;
-        const long q0 = (long)iz * ld + ixb;
-        const int a1 = last_x ? NP : N;
-        if constexpr (FORCE_ONLY) {
+        /* next row's loads go out before this row's stores */
+        RowFetch<N> nxt;
+        if constexpr (AHEAD) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (ez + 1 < ez1) fetch(ez + 1, nxt);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (own) {
+            const int ixb = ex * N;
+            const long q0 = (long)iz * ld + ixb;
+            const int a1 = last_x ? NP : N;
+            const bool vec = N == 4 && b < N && iz != 0 && ex != 0;
+            if constexpr (FORCE_ONLY) {
 #pragma unroll
-            for (int a = 0; a < NP; ++a)
-                if (a < a1)
+                for (int a = 0; a < NP; ++a)
+                    if (a < a1)
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) gp(D.fout[k])[q0 + a] = f[a][k];
-            continue;
-        } else {
-            if constexpr (N == 4) {
-                if (vec) {
-                    float4 un[3];
-                    uint4 cw = make_uint4(0u, 0u, 0u, 0u);
+                        for (int k = 0; k < 3; ++k) gp(D.fout[k])[q0 + a] = f[a][k];
+            } else if (N == 4 && vec) {
+                float4 un[3], dn[3];
+                uint4 cw = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        float d[4] = {pd[k].x, pd[k].y, pd[k].z, pd[k].w};
-                        float v[4];
+                for (int k = 0; k < 3; ++k) {
+                    float d[4] = {cur.pd[k].x, cur.pd[k].y, cur.pd[k].z, cur.pd[k].w};
+                    float v[4];
 #pragma unroll
-                        for (int a = 0; a < 4; ++a) {
-                            d[a] = d[a] - m[a] * f[a][k];
-                            v[a] = u[a][k];
-                            if (kahan) {
-                                const uint32_t wc = a == 0 ? pc.x : a == 1 ? pc.y : a == 2 ? pc.z : pc.w;
-                                float c = c_unpack(wc, k, v[a]);
-                                kahan_add(v[a], c, d[a]);
-                                const uint32_t bits = c_pack(c, v[a]) << (8 * k);
-                                if (a == 0) cw.x |= bits; else if (a == 1) cw.y |= bits; else if (a == 2) cw.z |= bits; else cw.w |= bits;
-                            } else {
-                                v[a] = v[a] + d[a];
-                            }
+                    for (int a = 0; a < 4; ++a) {
+                        d[a] = d[a] - cur.m[a] * f[a][k];
+                        v[a] = u[a][k];
+                        if (kahan) {
+                            const uint32_t wc = a == 0 ? cur.pc.x : a == 1 ? cur.pc.y : a == 2 ? cur.pc.z : cur.pc.w;
+                            float c = c_unpack(wc, k, v[a]);
+                            kahan_add(v[a], c, d[a]);
+                            const uint32_t bits = c_pack(c, v[a]) << (8 * k);
+                            if (a == 0) cw.x |= bits; else if (a == 1) cw.y |= bits; else if (a == 2) cw.z |= bits; else cw.w |= bits;
+                        } else {
+                            v[a] = v[a] + d[a];
                         }
-                        pd[k] = make_float4(d[0], d[1], d[2], d[3]);
-                        un[k] = make_float4(v[0], v[1], v[2], v[3]);
                     }
+                    dn[k] = make_float4(d[0], d[1], d[2], d[3]);
+                    un[k] = make_float4(v[0], v[1], v[2], v[3]);
+                }
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.del[k]) + q0) = pd[k];
-                        *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[p ^ 1][k]) + q0) = un[k];
-                    }
-                    if (kahan) *reinterpret_cast<SEMSKS_GMEM uint4 *>(gp(D.cmp) + q0) = cw;
-                    if (last_x) {
-                        float fa[3] = {f[N][0], f[N][1], f[N][2]};
-                        float ua[3] = {u[N][0], u[N][1], u[N][2]};
-                        update_one<N>(D, p, q0 + N, ixb + N, iz, MIXED ? (int)idrow[N] : rid, fa, ua);
-                    }
-                    continue;
+                for (int k = 0; k < 3; ++k) {
+                    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.del[k]) + q0) = dn[k];
+                    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[p ^ 1][k]) + q0) = un[k];
+                }
+                if (kahan) *reinterpret_cast<SEMSKS_GMEM uint4 *>(gp(D.cmp) + q0) = cw;
+                if (last_x) {
+                    float fa[3] = {f[N][0], f[N][1], f[N][2]};
+                    float ua[3] = {u[N][0], u[N][1], u[N][2]};
+                    update_one<N>(D, p, q0 + N, ixb + N, iz, MIXED ? (int)idrow[N] : rid, fa, ua);
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < NP; ++a) {
+                    if (a >= a1) continue;
+                    float fa[3] = {f[a][0], f[a][1], f[a][2]};
+                    float ua[3] = {u[a][0], u[a][1], u[a][2]};
+                    update_one<N>(D, p, q0 + a, ixb + a, iz, MIXED ? (int)idrow[a] : rid, fa, ua);
                 }
             }
-#pragma unroll
-            for (int a = 0; a < NP; ++a) {
-                if (a >= a1) continue;
-                float fa[3] = {f[a][0], f[a][1], f[a][2]};
-                float ua[3] = {u[a][0], u[a][1], u[a][2]};
-                update_one<N>(D, p, q0 + a, ixb + a, iz, MIXED ? (int)idrow[a] : rid, fa, ua);
-            }
         }
+        if constexpr (AHEAD) cur = nxt;
     }
 }
 
@@ -919,7 +957,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
    << 16) from the work list until it is exhausted; a mixed tile takes the
    per-node material path */
 template <int N, bool FORCE_ONLY>
-__global__ __launch_bounds__(256, 3) void step_kernel(const InstDesc *__restrict__ insts, const int *__restrict__ levels,
+__global__ __launch_bounds__(256, N <= 4 ? 3 : 2) void step_kernel(const InstDesc *__restrict__ insts, const int *__restrict__ levels,
                                                       const int4 *__restrict__ tiles, const int *__restrict__ ntiles)
 {
     __shared__ float lds_blocks[4][RowLayout<N>::WAVE_FLOATS];
