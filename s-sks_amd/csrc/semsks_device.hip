@@ -307,8 +307,8 @@ __global__ void init_state_kernel(InstDesc D)
             q = (long)iz * ld + nbl;
             u = g[2 * nx + iz]; w = -g[2 * nx + nz + iz]; if (sh) v = gv[nx + iz];
         }
-        D.u[0][0][q] = u; D.u[0][1][q] = v; D.u[0][2][q] = w;
-        D.del[0][q] = u; D.del[1][q] = v; D.del[2][q] = w;
+        D.u[0][3 * q] = u; D.u[0][3 * q + 1] = v; D.u[0][3 * q + 2] = w;
+        D.st[4 * q] = u; D.st[4 * q + 1] = v; D.st[4 * q + 2] = w;
     }
 }
 
@@ -358,7 +358,8 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
     const float *g = D.G + (long)lev * mt, *g0 = D.G + (long)(lev - 1) * mt;
     const float *gv = D.Gv + (long)lev * sht, *gv0 = D.Gv + (long)(lev - 1) * sht;
     const bool sh = D.solver == SEMSKS_SOLVER_PSVSH, mp = D.solver == SEMSKS_SOLVER_PSV_MP;
-    float *U[3] = {D.u[p][0], D.u[p][1], D.u[p][2]};
+    float *U = D.u[p];
+    float *S = D.st;
     const int nb = nx - nbl, nl = nz - nbl - 1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb + nl; i += gridDim.x * blockDim.x) {
         long q;
@@ -367,9 +368,9 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
             int ix = nbl + i;
             q = (long)nbl * ld + ix;
             if (mp) {
-                d[0] = -(U[0][q] - D.del[0][q]) + g[ix];
+                d[0] = -(U[3 * q] - S[4 * q]) + g[ix];
                 d[1] = 0.f;
-                d[2] = -(U[2][q] - D.del[2][q]) - g[ix + nx];
+                d[2] = -(U[3 * q + 2] - S[4 * q + 2]) - g[ix + nx];
             } else {
                 d[0] = -g0[ix] + g[ix];
                 d[1] = sh ? -gv0[ix] + gv[ix] : 0.f;
@@ -379,19 +380,19 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
             int iz = nbl + 1 + (i - nb);
             q = (long)iz * ld + nbl;
             if (mp) {
-                d[0] = -(U[0][q] - D.del[0][q]) + g[2 * nx + iz];
+                d[0] = -(U[3 * q] - S[4 * q]) + g[2 * nx + iz];
                 d[1] = 0.f;
-                d[2] = -(U[2][q] - D.del[2][q]) - g[2 * nx + nz + iz];
+                d[2] = -(U[3 * q + 2] - S[4 * q + 2]) - g[2 * nx + nz + iz];
             } else {
                 d[0] = -g0[2 * nx + iz] + g[2 * nx + iz];
                 d[1] = sh ? -gv0[nx + iz] + gv[nx + iz] : 0.f;
                 d[2] = g0[2 * nx + nz + iz] - g[2 * nx + nz + iz];
             }
         }
-        const uint32_t w = D.kahan ? D.cmp[q] : 0u;
+        const uint32_t w = D.kahan ? __float_as_uint(S[4 * q + 3]) : 0u;
         uint32_t wn = 0;
         for (int k = 0; k < 3; ++k) {
-            float u = U[k][q];
+            float u = U[3 * q + k];
             if (D.kahan) {
                 float c = c_unpack(w, k, u);
                 kahan_add(u, c, d[k]);
@@ -399,10 +400,10 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
             } else {
                 u = u + d[k];
             }
-            U[k][q] = u;
-            D.del[k][q] = D.del[k][q] + d[k];
+            U[3 * q + k] = u;
+            S[4 * q + k] = S[4 * q + k] + d[k];
         }
-        if (D.kahan) D.cmp[q] = wn;
+        if (D.kahan) S[4 * q + 3] = __uint_as_float(wn);
     }
 }
 
@@ -417,9 +418,9 @@ __global__ void record_kernel(const InstDesc *insts, int *levels)
     float *tr = D.traces + (long)lev * 3 * nt;
     for (int r = threadIdx.x; r < nt; r += blockDim.x) {
         long q = D.rec[r];
-        tr[r] = D.u[p][0][q];
-        tr[nt + r] = -D.u[p][2][q];
-        tr[2 * nt + r] = D.u[p][1][q];
+        tr[r] = D.u[p][3 * q];
+        tr[nt + r] = -D.u[p][3 * q + 2];
+        tr[2 * nt + r] = D.u[p][3 * q + 1];
     }
     __syncthreads();
     if (threadIdx.x == 0) levels[blockIdx.x] = lev;
@@ -430,9 +431,9 @@ __global__ void record0_kernel(InstDesc D)
     const int nt = D.ntrace;
     for (int r = threadIdx.x; r < nt; r += blockDim.x) {
         long q = D.rec[r];
-        D.traces[r] = D.u[0][0][q];
-        D.traces[nt + r] = -D.u[0][2][q];
-        D.traces[2 * nt + r] = D.u[0][1][q];
+        D.traces[r] = D.u[0][3 * q];
+        D.traces[nt + r] = -D.u[0][3 * q + 2];
+        D.traces[2 * nt + r] = D.u[0][3 * q + 1];
     }
 }
 
@@ -541,9 +542,9 @@ __device__ __forceinline__ void update_one(const InstDesc &D, int p, long q, int
     const bool kahan = D.kahan != 0;
     const int e = edge_index(ix, iz, D.nx, D.nz, D.hpn);
     float d[3], ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) d[k] = gp(D.del[k])[q];
-    const uint32_t w = kahan ? gp(D.cmp)[q] : 0u;
+    const float4 s4 = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.st) + 4 * q);
+    d[0] = s4.x; d[1] = s4.y; d[2] = s4.z;
+    const uint32_t w = kahan ? __float_as_uint(s4.w) : 0u;
     float m = 0.f;
     if (e >= 0) {
 #pragma unroll
@@ -565,12 +566,9 @@ __device__ __forceinline__ void update_one(const InstDesc &D, int p, long q, int
             nu[k] = nu[k] + d[k];
         }
     }
+    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.st) + 4 * q) = make_float4(d[0], d[1], d[2], __uint_as_float(wn));
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        gp(D.del[k])[q] = d[k];
-        gp(D.u[p ^ 1][k])[q] = nu[k];
-    }
-    if (kahan) gp(D.cmp)[q] = wn;
+    for (int k = 0; k < 3; ++k) gp(D.u[p ^ 1])[3 * q + k] = nu[k];
 }
 
 /* Lane layout of the element-force/update kernel: a 64-lane wave holds
@@ -725,8 +723,7 @@ __device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)
 template <int N> struct RowFetch {
     float u[N][3];
     float un[3];
-    float4 pd[3];
-    uint4 pc;
+    float4 sd[4];      /* per owned node: delta xyz, packed compensation */
     float m[4];
     int rid;           /* single-record tiles: the row's material record */
 };
@@ -756,9 +753,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     const int ex_first = x0 > 0 ? x0 - 1 : 0;
     const RowCoef<N> rc = row_coef<N>(b);
 
-    const SEMSKS_GMEM float *U0 = gp(FORCE_ONLY ? D.u[0][0] : D.u[p][0]);
-    const SEMSKS_GMEM float *U1 = gp(FORCE_ONLY ? D.u[0][1] : D.u[p][1]);
-    const SEMSKS_GMEM float *U2 = gp(FORCE_ONLY ? D.u[0][2] : D.u[p][2]);
+    const SEMSKS_GMEM float *U = gp(FORCE_ONLY ? D.u[0] : D.u[p]);
 
     const int ez0 = z0 > 0 ? z0 - 1 : 0;
     const int ez1 = z0 + rows;
@@ -766,36 +761,41 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     auto fetch = [&](int ez, RowFetch<N> &r) {
         const int iz = ez * N + b;
         const long rowoff = (long)iz * ld + (long)exl * N;
-        const SEMSKS_GMEM float *Uk[3] = {U0, U1, U2};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (valid) {
-                if constexpr (N == 4) {
-                    const float4 t = *reinterpret_cast<const SEMSKS_GMEM float4 *>(Uk[k] + rowoff);
-                    r.u[0][k] = t.x; r.u[1][k] = t.y; r.u[2][k] = t.z; r.u[3][k] = t.w;
-                } else {
-#pragma unroll
-                    for (int a = 0; a < N; ++a) r.u[a][k] = Uk[k][rowoff + a];
-                }
+        /* N nodes x 3 components, contiguous; 16-byte aligned for N = 4 */
+        if (valid) {
+            if constexpr (N == 4) {
+                const SEMSKS_GMEM float4 *v4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(U + 3 * rowoff);
+                const float4 t0 = v4[0], t1 = v4[1], t2 = v4[2];
+                r.u[0][0] = t0.x; r.u[0][1] = t0.y; r.u[0][2] = t0.z; r.u[1][0] = t0.w;
+                r.u[1][1] = t1.x; r.u[1][2] = t1.y; r.u[2][0] = t1.z; r.u[2][1] = t1.w;
+                r.u[2][2] = t2.x; r.u[3][0] = t2.y; r.u[3][1] = t2.z; r.u[3][2] = t2.w;
             } else {
 #pragma unroll
-                for (int a = 0; a < N; ++a) r.u[a][k] = 0.f;
+                for (int a = 0; a < N; ++a)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) r.u[a][k] = U[3 * (rowoff + a) + k];
             }
-            r.un[k] = direct ? Uk[k][rowoff + N] : 0.f;
+        } else {
+#pragma unroll
+            for (int a = 0; a < N; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) r.u[a][k] = 0.f;
         }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r.un[k] = direct ? U[3 * (rowoff + N) + k] : 0.f;
         r.rid = 0;
         if constexpr (!MIXED) r.rid = __builtin_amdgcn_readfirstlane((int)gp(D.emat)[(long)ez * nex + ex_first]);
         if constexpr (N == 4 && !FORCE_ONLY) {
             const bool own = own_x && ez >= z0 && b < N;
             const bool vec = own && iz != 0 && ex != 0;
-            r.pd[0] = r.pd[1] = r.pd[2] = make_float4(0.f, 0.f, 0.f, 0.f);
-            r.pc = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) r.sd[a] = make_float4(0.f, 0.f, 0.f, 0.f);
             r.m[0] = r.m[1] = r.m[2] = r.m[3] = 0.f;
             if (vec) {
                 const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
+                const SEMSKS_GMEM float4 *s4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.st) + 4 * rowoff);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) r.pd[k] = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.del[k]) + rowoff);
-                if (kahan) r.pc = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(gp(D.cmp) + rowoff);
+                for (int a = 0; a < 4; ++a) r.sd[a] = s4[a];
                 /* mass class: cz = b (0 = row shared with the element below) */
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
@@ -905,35 +905,34 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
 #pragma unroll
                         for (int k = 0; k < 3; ++k) gp(D.fout[k])[q0 + a] = f[a][k];
             } else if (N == 4 && vec) {
-                float4 un[3], dn[3];
-                uint4 cw = make_uint4(0u, 0u, 0u, 0u);
+                float v[4][3];
+                float4 sn[4];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    float d[4] = {cur.pd[k].x, cur.pd[k].y, cur.pd[k].z, cur.pd[k].w};
-                    float v[4];
+                for (int a = 0; a < 4; ++a) {
+                    float d[3] = {cur.sd[a].x, cur.sd[a].y, cur.sd[a].z};
+                    const uint32_t wc = __float_as_uint(cur.sd[a].w);
+                    uint32_t cw = 0u;
 #pragma unroll
-                    for (int a = 0; a < 4; ++a) {
-                        d[a] = d[a] - cur.m[a] * f[a][k];
-                        v[a] = u[a][k];
+                    for (int k = 0; k < 3; ++k) {
+                        d[k] = d[k] - cur.m[a] * f[a][k];
+                        v[a][k] = u[a][k];
                         if (kahan) {
-                            const uint32_t wc = a == 0 ? cur.pc.x : a == 1 ? cur.pc.y : a == 2 ? cur.pc.z : cur.pc.w;
-                            float c = c_unpack(wc, k, v[a]);
-                            kahan_add(v[a], c, d[a]);
-                            const uint32_t bits = c_pack(c, v[a]) << (8 * k);
-                            if (a == 0) cw.x |= bits; else if (a == 1) cw.y |= bits; else if (a == 2) cw.z |= bits; else cw.w |= bits;
+                            float c = c_unpack(wc, k, v[a][k]);
+                            kahan_add(v[a][k], c, d[k]);
+                            cw |= c_pack(c, v[a][k]) << (8 * k);
                         } else {
-                            v[a] = v[a] + d[a];
+                            v[a][k] = v[a][k] + d[k];
                         }
                     }
-                    dn[k] = make_float4(d[0], d[1], d[2], d[3]);
-                    un[k] = make_float4(v[0], v[1], v[2], v[3]);
+                    sn[a] = make_float4(d[0], d[1], d[2], __uint_as_float(cw));
                 }
+                SEMSKS_GMEM float4 *s4 = reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.st) + 4 * q0);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.del[k]) + q0) = dn[k];
-                    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[p ^ 1][k]) + q0) = un[k];
-                }
-                if (kahan) *reinterpret_cast<SEMSKS_GMEM uint4 *>(gp(D.cmp) + q0) = cw;
+                for (int a = 0; a < 4; ++a) s4[a] = sn[a];
+                SEMSKS_GMEM float4 *u4 = reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[p ^ 1]) + 3 * q0);
+                u4[0] = make_float4(v[0][0], v[0][1], v[0][2], v[1][0]);
+                u4[1] = make_float4(v[1][1], v[1][2], v[2][0], v[2][1]);
+                u4[2] = make_float4(v[2][2], v[3][0], v[3][1], v[3][2]);
                 if (last_x) {
                     float fa[3] = {f[N][0], f[N][1], f[N][2]};
                     float ua[3] = {u[N][0], u[N][1], u[N][2]};
@@ -1310,8 +1309,7 @@ static int nt_sem_of(const semsks_problem *p) { return (int)((float)((p->nt - 3)
 size_t semsks_footprint(const semsks_problem *p, int shared)
 {
     const size_t nodes = (size_t)ld_of(p) * nz_of(p);
-    const int kahan = !(p->flags & SEMSKS_F_NO_KAHAN);
-    size_t b = nodes * 4 * (6 + 3 + (kahan ? 1 : 0));                  /* u x2, delta, packed comp */
+    size_t b = nodes * 4 * (6 + 4);                 /* u(t), u(t+dt), delta + packed compensation */
     if (!shared) b += nodes * 2 + (size_t)p->nex * p->nez * 2 +
                       (size_t)p->n_mat * (64 + 4 * (p->hpn + 1) * (p->hpn + 1)) +
                       (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24;
@@ -1400,12 +1398,12 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
 #define ACHK(x) do { if ((x) != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "device allocation failed"); goto fail; } } while (0)
     {
         float *st_all = nullptr;
-        const int nplanes = 6 + 3 + (d.kahan ? 1 : 0);
+        const int nplanes = 6 + 4;
         ACHK(dalloc(s, &st_all, nodes * nplanes));
         ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->stream));
-        for (int b = 0; b < 2; ++b) for (int c = 0; c < 3; ++c) d.u[b][c] = st_all + (b * 3 + c) * nodes;
-        for (int c = 0; c < 3; ++c) d.del[c] = st_all + (6 + c) * nodes;
-        d.cmp = d.kahan ? reinterpret_cast<uint32_t *>(st_all + 9 * nodes) : nullptr;
+        d.u[0] = st_all;
+        d.u[1] = st_all + 3 * nodes;
+        d.st = st_all + 6 * nodes;
     }
     if (share >= 0) {
         const InstDesc &o = e->slots[share].d;
@@ -1661,10 +1659,13 @@ semsks_status semsks_get_field(semsks_engine *e, int k, float *out)
     if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
     const Slot &s = e->slots[k];
     const int p = e->h_levels[k] & 1;
-    for (int c = 0; c < 3; ++c)
-        HIPCHK(hipMemcpy2DAsync(out + (size_t)c * s.nx * s.nz, s.nx * 4, s.d.u[p][c], s.ld * 4, s.nx * 4, s.nz,
-                                hipMemcpyDeviceToHost, e->stream));
+    std::vector<float> aos((size_t)s.ld * s.nz * 3);
+    HIPCHK(hipMemcpyAsync(aos.data(), s.d.u[p], aos.size() * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    for (int c = 0; c < 3; ++c)
+        for (int iz = 0; iz < s.nz; ++iz)
+            for (int ix = 0; ix < s.nx; ++ix)
+                out[((size_t)c * s.nz + iz) * s.nx + ix] = aos[((size_t)iz * s.ld + ix) * 3 + c];
     return SEMSKS_OK;
 }
 
@@ -1677,12 +1678,14 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     HIPCHK(hipMalloc(&buf, nodes * 6 * sizeof(float) + 256));
     HIPCHK(hipMemsetAsync(buf, 0, nodes * 6 * sizeof(float), e->stream));
     InstDesc d = s.d;
-    for (int c = 0; c < 3; ++c) {
-        d.u[0][c] = buf + c * nodes;
-        d.fout[c] = buf + (3 + c) * nodes;
-        HIPCHK(hipMemcpy2DAsync(d.u[0][c], s.ld * 4, u + (size_t)c * s.nx * s.nz, s.nx * 4, s.nx * 4, s.nz,
-                                hipMemcpyHostToDevice, e->stream));
-    }
+    std::vector<float> aos(nodes * 3, 0.f);
+    for (int c = 0; c < 3; ++c)
+        for (int iz = 0; iz < s.nz; ++iz)
+            for (int ix = 0; ix < s.nx; ++ix)
+                aos[((size_t)iz * s.ld + ix) * 3 + c] = u[((size_t)c * s.nz + iz) * s.nx + ix];
+    d.u[0] = buf;
+    for (int c = 0; c < 3; ++c) d.fout[c] = buf + (3 + c) * nodes;
+    HIPCHK(hipMemcpyAsync(d.u[0], aos.data(), aos.size() * 4, hipMemcpyHostToDevice, e->stream));
     d.active = 1;
     InstDesc *dd = nullptr;
     int4 *tl = nullptr;

@@ -29,10 +29,12 @@
 struct InstDesc {
     int nex, nez, nx, nz, ld;
     int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active;
-    float *u[2][3];
-    float *del[3];
-    uint32_t *cmp;
-    float *fout[3];                 /* force-only mode output, else null */
+    /* displacement u(t) and u(t+dt), xyz interleaved: node q at u[p][3q..3q+2] */
+    float *u[2];
+    /* update state, one 16-byte record per node: delta = u(t) - u(t-dt)
+       (xyz) and the packed Kahan compensation of u (bits) */
+    float *st;
+    float *fout[3];                 /* force-only mode output planes, else null */
     const uint16_t *mat_id;
     const uint16_t *emat;
     const float *mat;
