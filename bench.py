@@ -107,12 +107,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=32,
+                    help="untimed steps first (the tile autotuner and the hipGraph capture run here)")
     ap.add_argument("--config", default="c3")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--only", choices=["plain", "batched"], default=None,
+                    help="c3: run only one of the two legs (for profiler runs)")
     ap.add_argument("--kernel-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one device copy of this many MiB (PMC byte-counter calibration)")
@@ -138,12 +141,17 @@ def main():
     out = {}
     if args.config == "c3":
         # plain: one instance of the production anisotropic model (C2)
-        log("plain: C2 mesh, 1 instance")
-        b2 = configs.Built(configs.c2_case(nst=10))
-        r2 = run_engine([(b2, 0.05, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z, extra_steps=args.kernel_steps)
-        k2, _ = kernel_time(r2["engine"], args.kernel_steps)
-        r2["engine"].close()
-        log("  plain %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms" % (r2["throughput"], r2["ms_per_step"], k2))
+        r2, k2 = {"throughput": float("nan"), "ms_per_step": float("nan")}, float("nan")
+        if args.only != "batched":
+            log("plain: C2 mesh, 1 instance")
+            b2 = configs.Built(configs.c2_case(nst=10))
+            r2 = run_engine([(b2, 0.05, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z,
+                            extra_steps=args.kernel_steps)
+            k2, _ = kernel_time(r2["engine"], args.kernel_steps)
+            r2["engine"].close()
+            log("  plain %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms" % (r2["throughput"], r2["ms_per_step"], k2))
+            if args.only == "plain":
+                return
         # batched: 64 waves on the C2 mesh with the isotropic model, one material
         log("batched: C3, %d instances" % args.batch)
         b3 = configs.Built(configs.c3_case(nst=10))
@@ -152,6 +160,17 @@ def main():
         k3, _ = kernel_time(r3["engine"], args.kernel_steps)
         r3["engine"].close()
         log("  batched %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms" % (r3["throughput"], r3["ms_per_step"], k3))
+        # the same two legs on the 200x100 mesh (321,201 points per instance)
+        r1p = r1b = None
+        if args.only is None:
+            log("plain and batched on the C1 mesh")
+            b1 = configs.Built(configs.c1_case(nst=10, solver=0, sh_zero=False))
+            r1p = run_engine([(b1, 0.06, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z)
+            r1p["engine"].close()
+            w1 = [(b1, p, baz) for p, baz in configs.c3_waves()][: args.batch]
+            r1b = run_engine(w1, args.steps, args.warmup, True, tile_z=args.tile_z)
+            r1b["engine"].close()
+            log("  C1 plain %.3e, batched x%d %.3e pt-steps/s" % (r1p["throughput"], len(w1), r1b["throughput"]))
         B = alg_bytes(9, len(waves))
         achieved = B * r3["npts"] / (k3 * 1e-3) / 1e9
         value, ms, npts, steps = r3["throughput"], r3["ms_per_step"], r3["npts"], args.steps
@@ -163,6 +182,10 @@ def main():
                "batched_over_plain": r3["throughput"] / r2["throughput"],
                "kernel_ms_batched": k3, "kernel_ms_plain": k2,
                "bytes_per_point_step_alg": B, "kahan": True}
+        if r1p is not None:
+            cfg.update({"c1_mesh_plain_value": r1p["throughput"], "c1_mesh_batched_value": r1b["throughput"],
+                        "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
+                        "c1_mesh_workload": "64 plane S waves on the 200x100-element mesh vs one"})
         cpu_case = lambda: configs.c2_case(nst=10)
     elif args.config == "c2":
         b2 = configs.Built(configs.c2_case(nst=10))

@@ -32,6 +32,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <vector>
+#include <map>
 #include <algorithm>
 
 #include "semsks.h"
@@ -1108,6 +1109,11 @@ struct semsks_engine {
     std::vector<hipEvent_t> tev;      /* per-step timing events */
     int timed_steps = 0;
     double last_ms[2] = {0, 0};
+    /* tile height in element rows: opts.tile_z when set, else the
+       autotuner's choice for the current set of instances */
+    int tz_cur = 8;
+    bool tuned = false;
+    std::map<uint64_t, int> tz_cache;   /* instance-set signature -> tile height */
     /* device-resident plane-wave records waiting for the next admit */
     float *pend_psv = nullptr, *pend_sh = nullptr;
     int pend_nt = 0, pend_nx = 0, pend_nz = 0;
@@ -1115,8 +1121,23 @@ struct semsks_engine {
 
 static int tile_rows(const semsks_engine *e, int nez)
 {
-    int tz = e->opts.tile_z > 0 ? e->opts.tile_z : 8;
+    int tz = e->opts.tile_z > 0 ? e->opts.tile_z : e->tz_cur;
     return std::min(tz, std::max(1, nez));
+}
+
+static void drop_graph(semsks_engine *e);
+
+/* What the best tile height depends on: degree mix, how many instances
+   share the step and how many element columns and rows they have (both
+   rounded to powers of two). */
+static uint64_t tune_signature(const semsks_engine *e)
+{
+    uint64_t nsl = 0, ncol = 0, nrow = 0;
+    for (auto &s : e->slots)
+        if (s.used) { ++nsl; ncol += s.meta.nex; nrow += s.meta.nez; }
+    auto lg = [](uint64_t v) { uint64_t l = 0; while (v > 1) { v >>= 1; ++l; } return l; };
+    return (uint64_t)e->hpn_mask | (lg(nsl) << 8) | (lg(ncol / std::max<uint64_t>(nsl, 1)) << 16) |
+           (lg(nrow / std::max<uint64_t>(nsl, 1)) << 24);
 }
 
 static semsks_status upload_tables(semsks_engine *e)
@@ -1134,7 +1155,7 @@ static semsks_status upload_tables(semsks_engine *e)
     std::vector<int4> t;
     int cnt[SEMSKS_MAXNP] = {0};
     e->hpn_mask = 0;
-    const int tzg = e->opts.tile_z > 0 ? e->opts.tile_z : 8;
+    const int tzg = e->opts.tile_z > 0 ? e->opts.tile_z : e->tz_cur;
     for (int n = 3; n <= 6; ++n) {
         e->tile_off[n] = (int)t.size();
         int maxx = 0, maxz = 0;
@@ -1156,6 +1177,8 @@ static semsks_status upload_tables(semsks_engine *e)
         if (cnt[n]) e->hpn_mask |= 1u << n;
     }
     if ((int)t.size() > e->tile_cap) {
+        drop_graph(e);                /* a captured step holds the old list */
+        HIPCHK(hipStreamSynchronize(e->stream));
         if (e->d_tiles) HIPCHK(hipFree(e->d_tiles));
         e->tile_cap = (int)t.size() + 1024;
         HIPCHK(hipMalloc(&e->d_tiles, e->tile_cap * sizeof(int4)));
@@ -1165,6 +1188,14 @@ static semsks_status upload_tables(semsks_engine *e)
     HIPCHK(hipMemcpyAsync(e->d_ntiles, cnt, sizeof cnt, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     e->dirty = false;
+    if (e->opts.tile_z <= 0) {
+        auto it = e->tz_cache.find(tune_signature(e));
+        e->tuned = it != e->tz_cache.end();
+        if (e->tuned && it->second != tzg) {
+            e->tz_cur = it->second;
+            return upload_tables(e);
+        }
+    }
     return SEMSKS_OK;
 }
 
@@ -1572,10 +1603,45 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     HIPCHK(hipEventRecord(e->ev[0], e->stream));
     double kern_ms = 0.0;
     e->timed_steps = 0;
+    int done = 0;
+    /* autotuner: with tile_z unset, the first call that has the steps to
+       spare runs two real steps at each candidate tile height and keeps the
+       fastest (the tiling does not change the arithmetic, so these steps
+       are ordinary steps of the run) */
+    static const int cand[] = {4, 8, 16, 32};
+    const int ncand = (int)(sizeof cand / sizeof cand[0]);
+    if (e->opts.tile_z <= 0 && !e->tuned && nsteps >= 4 * ncand && sync != 2) {
+        while ((int)e->tev.size() < 4) {
+            hipEvent_t v;
+            HIPCHK(hipEventCreate(&v));
+            e->tev.push_back(v);
+        }
+        int best = e->tz_cur;
+        float best_ms = 1e30f;
+        for (int c = 0; c < ncand; ++c) {
+            e->tz_cur = cand[c];
+            semsks_status st = upload_tables(e);
+            if (st) return st;
+            for (int r = 0; r < 2; ++r) {
+                st = launch_step(e, e->tev[2 * r], e->tev[2 * r + 1]);
+                if (st) return st;
+            }
+            HIPCHK(hipEventSynchronize(e->tev[3]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e->tev[2], e->tev[3]));
+            if (ms < best_ms) { best_ms = ms; best = cand[c]; }
+            done += 2;
+        }
+        e->tz_cur = best;
+        semsks_status st = upload_tables(e);
+        if (st) return st;
+        e->tz_cache[tune_signature(e)] = best;
+        e->tuned = true;
+        drop_graph(e);
+    }
     if (e->opts.use_graph && sync != 2) {
         const int G = 16;
-        int done = 0;
-        if (nsteps >= G) {
+        if (nsteps - done >= G) {
             if (!e->gexec || e->graph_steps != G || e->graph_mask != e->hpn_mask) {
                 semsks_status st = build_graph(e, G);
                 if (st) return st;
@@ -1586,14 +1652,15 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     } else {
         /* eager launches with an event pair around every step-kernel launch
            (the first 1024 steps of the call are timed) */
-        const int ntimed = sync ? std::min(nsteps, 1024) : 0;
+        const int ntimed = sync ? std::min(nsteps - done, 1024) : 0;
         while ((int)e->tev.size() < 2 * ntimed) {
             hipEvent_t v;
             HIPCHK(hipEventCreate(&v));
             e->tev.push_back(v);
         }
-        for (int i = 0; i < nsteps; ++i) {
-            semsks_status st = i < ntimed ? launch_step(e, e->tev[2 * i], e->tev[2 * i + 1]) : launch_step(e, nullptr, nullptr);
+        for (int i = done; i < nsteps; ++i) {
+            const int j = i - done;
+            semsks_status st = j < ntimed ? launch_step(e, e->tev[2 * j], e->tev[2 * j + 1]) : launch_step(e, nullptr, nullptr);
             if (st) return st;
         }
         if (ntimed) {

@@ -171,3 +171,16 @@ def test_dipping_interface_and_tilted_axis(tmp_path, monkeypatch):
     ref = o.run()
     for k, (l2, pk) in enumerate(misfits(tr, ref)):
         assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, l2, pk)
+
+
+def test_tile_height_and_autotuner_leave_results_unchanged(tmp_path, monkeypatch):
+    """The element sweep's arithmetic does not depend on how the mesh is cut
+    into tiles, so fixed tile heights and the autotuned one (which runs its
+    trial steps as ordinary steps) give bit-identical traces."""
+    print("case: small, tile heights 3 / 16 / autotuned", flush=True)
+    d = make_case(tmp_path, psi_deg=30.0, sh_zero=False, interfaces=[([0.0, 50.0], [14.0, 18.0]), None, None])
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    runs = [run_gpu(m.problem, tile_z=tz) for tz in (3, 16, 0)]
+    for r in runs[1:]:
+        assert np.array_equal(r.view(np.uint32), runs[0].view(np.uint32))
