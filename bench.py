@@ -42,6 +42,20 @@ def alg_bytes(n_c: int, share: int) -> float:
     return 36.0 + 4.0 * (n_c + 1) / share
 
 
+def profiled_traffic(name: str):
+    """HBM bytes per launch of the step kernel for this workload, from the
+    rocprofv3 PMC summary kept under profiles/ (request-size counters of the
+    L2-to-fabric interface; see that file for the commands)."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -187,6 +201,7 @@ def main():
                         "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
                         "c1_mesh_workload": "64 plane S waves on the 200x100-element mesh vs one"})
         cpu_case = lambda: configs.c2_case(nst=10)
+        traffic = profiled_traffic("step_kernel_batched64_2000x500.md") if len(waves) == 64 else None
     elif args.config == "c2":
         b2 = configs.Built(configs.c2_case(nst=10))
         r2 = run_engine([(b2, 0.05, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z, extra_steps=args.kernel_steps)
@@ -198,6 +213,7 @@ def main():
         cfg = {"workload": "1 instance, 2000x500 mesh, tilted-axis hexagonal 4% upper mantle", "instances": 1,
                "kernel_ms": k2, "bytes_per_point_step_alg": B}
         cpu_case = lambda: configs.c2_case(nst=10)
+        traffic = profiled_traffic("step_kernel_plain_2000x500.md")
     elif args.config == "c1b":
         # the literal reading: 64 waves on the C1 mesh (321,201 points each)
         b1 = configs.Built(configs.c1_case(nst=10, solver=0, sh_zero=False))
@@ -215,6 +231,7 @@ def main():
                "plain_value": r1["throughput"], "batched_over_plain": rb["throughput"] / r1["throughput"],
                "kernel_ms_batched": kb, "kernel_ms_plain": k1, "bytes_per_point_step_alg": B}
         cpu_case = lambda: configs.c1_case(nst=10)
+        traffic = None
     else:
         raise SystemExit("unknown config " + args.config)
 
@@ -235,7 +252,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (closed-form plane S waves, seeded models)",
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -16,15 +16,23 @@
  * which is algebraically the reference's u+ = -(dt^2/M) f + 2u - u- and its
  * absorbing rows; u_{n+1} = u_n + delta_n is a Kahan-compensated sum.
  *
- * Element force: one 64-lane wave owns a tile of 63 element columns x TZ
- * element rows.  Lane L computes element column x0-1+L (lane 0 is a halo
- * column whose only output is its shared right column), sweeping the rows
- * upward from z0-1 (a halo row whose only output is its shared top row).
- * Shared nodes are assembled in registers: the top row of an element is
- * carried to the next row, the right column is handed to the next lane with
- * a cross-lane shift.  Every node is written by exactly one lane, so the
- * assembly needs neither atomics nor a colouring pass, and the update is
- * applied in the same kernel while the element's u is still in registers.
+ * Element force (row-per-lane layout): a 64-lane wave owns a tile of
+ * NE-1 = 64/(N+1)-1 element columns (11 for N = 4) x TZ element rows.  The
+ * wave holds NE element columns side by side, N+1 lanes per element, one
+ * lane per node row of the element; column 0 is a halo whose only output is
+ * its shared right node column.  The wave sweeps the element rows upward
+ * from z0-1 (a halo row whose only output is its shared top node row).
+ * x-derivatives and row tractions stay inside a lane; z-derivatives and
+ * column tractions go through a small per-wave LDS block.  Shared nodes are
+ * assembled without atomics or colouring: the raw top-row force of an
+ * element is carried to the next element row through LDS, the right node
+ * column is handed to the next element with a cross-lane shift, and every
+ * node is written by exactly one lane, which applies the update in the same
+ * pass while the row of u is still in registers.
+ *
+ * State in HBM: u(t) and u(t+dt) as xyz-interleaved rows (12 bytes per
+ * node), and one 16-byte record per node holding delta (xyz) and the packed
+ * Kahan compensation of u.  Rows are padded to a multiple of 64 nodes.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
