@@ -15,8 +15,14 @@ roofline    = algorithmic bytes per launch of the fused element-force/update
               stream), against MI355X's 8 TB/s HBM
 cpu_baseline= the faithful FP32 port (oracle/sem_oracle.c, -O3, OpenMP node
               loops as the reference) on a bounded sample of the same mesh
+serving     = continuous batching (semsks_serve) of Poisson-arriving requests:
+              p50/p99 end-to-end latency and throughput (default run: 20/s
+              for 5 s of scaled C4-style requests; --config c5: 20/s for
+              300 s of full-size ones)
+traffic     = measured HBM bytes per launch, from the rocprofv3 summary in
+              profiles/ for this workload
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3|c2|c1b|c4]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3|c2|c1b|c5]
 """
 from __future__ import annotations
 
@@ -86,6 +92,38 @@ def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0):
     return dict(throughput=npts * steps / wall, ms_per_step=1e3 * wall / steps, npts=npts, engine=eng, slots=slots)
 
 
+def run_serving(rate, duration, nex_range, nez_range, nst_range, budget_gb=200.0, max_slots=1024, seed=None):
+    """Continuous batching of Poisson-arriving C4-style requests
+    (semsks_serve): HBM-footprint admission against budget_gb, all admitted
+    instances stepped together, each retired at its own last level."""
+    import semsks
+    from semsks import configs
+    kw = {} if seed is None else {"seed": seed}
+    reqs = configs.c5_requests(rate=rate, duration=duration, nex_range=nex_range, nez_range=nez_range,
+                               nst_range=nst_range, **kw)
+    t0 = time.time()
+    built = [configs.Built(c) for _, c in reqs]
+    probs, pws = [], []
+    for b in built:
+        pr = b.problem(b.case.nst + 3)
+        probs.append(pr)
+        a = b.planewave_args(pr.nt)
+        pws.append((a[7], a[8], a[9], a[10]))
+    log("  built %d requests in %.1fs" % (len(reqs), time.time() - t0))
+    budget = int(budget_gb * 1e9)
+    with semsks.Engine(max_slots=max_slots, hbm_budget=budget) as e:
+        res, st = e.serve(probs, [a for a, _ in reqs], pws, steps_per_round=16, want_traces=True)
+    lat = np.array([r["latency"] for r in res if r["status"] == 0])
+    wait = np.array([r["admit"] - r["arrival"] for r in res if r["status"] == 0])
+    return {"requests": len(reqs), "completed": st["completed"], "rate_per_s": rate, "arrival_window_s": duration,
+            "wall_s": st["wall_s"], "throughput": st["point_steps"] / st["wall_s"],
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "p50_admission_wait_s": float(np.percentile(wait, 50)), "max_active": st["max_active"],
+            "hbm_budget_gb": budget_gb,
+            "mesh_elements": "%dx%d..%dx%d" % (nex_range[0], nez_range[0], nex_range[1], nez_range[1]),
+            "steps": "%d..%d" % nst_range}
+
+
 def kernel_time(eng, nsteps):
     """Average device time of the fused step kernel over nsteps eager steps."""
     import semsks
@@ -128,6 +166,8 @@ def main():
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--serve-window", type=float, default=5.0,
+                    help="seconds of Poisson arrivals in the serving leg of the default run (0 = skip)")
     ap.add_argument("--only", choices=["plain", "batched"], default=None,
                     help="c3: run only one of the two legs (for profiler runs)")
     ap.add_argument("--kernel-steps", type=int, default=10)
@@ -196,6 +236,12 @@ def main():
                "batched_over_plain": r3["throughput"] / r2["throughput"],
                "kernel_ms_batched": k3, "kernel_ms_plain": k2,
                "bytes_per_point_step_alg": B, "kahan": True}
+        if args.only is None and args.serve_window > 0:
+            log("serving: Poisson 20/s for %.0f s, scaled C4-style requests" % args.serve_window)
+            sv = run_serving(20.0, args.serve_window, (200, 800), (100, 200), (500, 2000))
+            log("  %d requests, p50 %.3f s, p99 %.3f s, %.3e pt-steps/s" % (sv["completed"], sv["p50_latency_s"],
+                                                                         sv["p99_latency_s"], sv["throughput"]))
+            cfg["serving"] = sv
         if r1p is not None:
             cfg.update({"c1_mesh_plain_value": r1p["throughput"], "c1_mesh_batched_value": r1b["throughput"],
                         "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
@@ -214,6 +260,14 @@ def main():
                "kernel_ms": k2, "bytes_per_point_step_alg": B}
         cpu_case = lambda: configs.c2_case(nst=10)
         traffic = profiled_traffic("step_kernel_plain_2000x500.md")
+    elif args.config == "c5":
+        # the full serving configuration: 20/s for 300 s of C4-size requests
+        sv = run_serving(20.0, 300.0, (200, 1600), (100, 400), (2000, 12000))
+        value, ms, steps = sv["throughput"], float("nan"), 0
+        achieved, traffic = float("nan"), None
+        cfg = {"workload": "serving: Poisson 20 requests/s for 300 s, C4-style instances, 200 GB HBM budget",
+               "serving": sv}
+        cpu_case = lambda: configs.c2_case(nst=10)
     elif args.config == "c1b":
         # the literal reading: 64 waves on the C1 mesh (321,201 points each)
         b1 = configs.Built(configs.c1_case(nst=10, solver=0, sh_zero=False))
@@ -257,6 +311,9 @@ def main():
     }
     if cpu:
         line["config"]["vs_cpu"] = value / cpu["value"]
+    if "serving" in cfg:
+        line["latency_p50_s"] = cfg["serving"]["p50_latency_s"]
+        line["latency_p99_s"] = cfg["serving"]["p99_latency_s"]
     print(json.dumps(line), flush=True)
 
 

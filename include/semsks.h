@@ -182,6 +182,39 @@ semsks_status semsks_last_timing(semsks_engine *e, double out[2]);
 int semsks_last_timed_steps(semsks_engine *e);
 
 /* ---------------------------------------------------------------------- */
+/* Serving: continuous batching of arriving requests                       */
+/* ---------------------------------------------------------------------- */
+/* No counterpart in the reference, which runs one solve per process
+   (applications/test/prem_epi83.pl:81-82 starts sem2d_psvsh per event). */
+typedef struct semsks_request {
+    const semsks_problem *problem;  /* in: the instance */
+    double arrival_s;               /* in: seconds after semsks_serve starts; non-decreasing */
+    int use_planewave;              /* in: 1 = boundary records from semsks_planewave on the
+                                       device (problem->psv and ->sh must be NULL) */
+    double planewave[4];            /* in: p (s/km), beta (km/s), f0 (Hz), psi (deg) */
+    float *traces;                  /* out (optional): [nst][3][ntrace], u, -w, v */
+    double admit_s, done_s;         /* out: admission and completion times (s after start) */
+    int slot;                       /* out: the slot it ran in */
+    int status;                     /* out: SEMSKS_OK, or why it was not run */
+} semsks_request;
+
+typedef struct semsks_serve_stats {
+    double wall_s;                  /* start to last completion */
+    double point_steps;             /* sum over completed requests of nglob * (nst - 1) */
+    int completed, max_active, rounds;
+} semsks_serve_stats;
+
+/* Runs the requests to completion in real time: each is admitted as a slot
+   once it has arrived and its footprint fits the engine's HBM budget (first
+   come, first served), all admitted instances advance together
+   steps_per_round steps at a time (one persistent launch per step), and
+   each is retired -- traces copied out, memory freed -- as soon as it
+   reaches its last level.  A request that cannot fit even into an idle
+   engine ends with status ENOHBM/ENOSLOT. */
+semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int steps_per_round,
+                           semsks_serve_stats *stats);
+
+/* ---------------------------------------------------------------------- */
 /* Single-run convenience (what the CLI uses)                              */
 /* ---------------------------------------------------------------------- */
 /* Runs a whole problem: engine with one slot, nst levels, traces to host

@@ -1113,6 +1113,7 @@ struct semsks_engine {
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
     unsigned hpn_mask = 0, graph_mask = 0;
+    int graph_off[SEMSKS_MAXNP] = {0};  /* tile offsets the captured step was built with */
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> tev;      /* per-step timing events */
     int timed_steps = 0;
@@ -1278,6 +1279,7 @@ static semsks_status build_graph(semsks_engine *e, int steps)
     HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
     e->graph_steps = steps;
     e->graph_mask = e->hpn_mask;
+    memcpy(e->graph_off, e->tile_off, sizeof e->graph_off);
     return SEMSKS_OK;
 }
 
@@ -1650,7 +1652,8 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     if (e->opts.use_graph && sync != 2) {
         const int G = 16;
         if (nsteps - done >= G) {
-            if (!e->gexec || e->graph_steps != G || e->graph_mask != e->hpn_mask) {
+            if (!e->gexec || e->graph_steps != G || e->graph_mask != e->hpn_mask ||
+                memcmp(e->graph_off, e->tile_off, sizeof e->graph_off) != 0) {
                 semsks_status st = build_graph(e, G);
                 if (st) return st;
             }

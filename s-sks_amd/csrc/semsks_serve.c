@@ -1,0 +1,141 @@
+/*
+ * semsks_serve.c -- continuous-batching request loop over the engine.
+ *
+ * The reference runs one solve per process (prem_epi83.pl:81-82 starts
+ * sem2d_psvsh once per event); it has no serving loop.  This loop takes a
+ * stream of requests with arrival times, admits each one as a slot as soon as
+ * it has arrived and its device footprint fits the engine's HBM budget
+ * (first come, first served), keeps every admitted instance's wavefield
+ * resident, steps all of them together (one persistent launch per time
+ * step), and retires each as it reaches its own last level, which frees its
+ * memory for the requests still waiting.  Latency = completion - arrival.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "semsks.h"
+#include "semsks_internal.h"
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void sleep_s(double s)
+{
+    if (s <= 0) return;
+    struct timespec ts;
+    ts.tv_sec = (time_t)s;
+    ts.tv_nsec = (long)((s - (double)ts.tv_sec) * 1e9);
+    nanosleep(&ts, NULL);
+}
+
+static semsks_status try_admit(semsks_engine *e, semsks_request *r, int *slot)
+{
+    const semsks_problem *p = r->problem;
+    if (r->use_planewave) {
+        semsks_status st = semsks_planewave(e, p->nex, p->nez, p->hpn, p->neh, p->eh, p->nt, p->dp, r->planewave[0],
+                                            r->planewave[1], r->planewave[2], r->planewave[3], NULL, NULL);
+        if (st != SEMSKS_OK) return st;
+    }
+    return semsks_admit(e, p, -1, slot);
+}
+
+semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int steps_per_round,
+                           semsks_serve_stats *stats)
+{
+    if (!e || (n > 0 && !req) || steps_per_round < 1) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
+    for (int i = 1; i < n; ++i)
+        if (req[i].arrival_s < req[i - 1].arrival_s)
+            return semsks_set_error(SEMSKS_EINVAL, "requests must be sorted by arrival time");
+    for (int i = 0; i < n; ++i) {
+        if (!req[i].problem) return semsks_set_error(SEMSKS_EINVAL, "request %d has no problem", i);
+        if (req[i].use_planewave && (req[i].problem->psv || req[i].problem->sh))
+            return semsks_set_error(SEMSKS_EINVAL, "request %d: plane-wave boundary and host records both given", i);
+        req[i].admit_s = req[i].done_s = -1.0;
+        req[i].slot = -1;
+        req[i].status = SEMSKS_OK;
+    }
+    int *queue = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    int *active = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    if (!queue || !active) {
+        free(queue); free(active);
+        return semsks_set_error(SEMSKS_EINVAL, "out of host memory");
+    }
+    int qh = 0, qt = 0, nact = 0, next = 0, finished = 0, max_active = 0, rounds = 0;
+    double point_steps = 0.0;
+    semsks_status st = SEMSKS_OK;
+    const double t0 = now_s();
+    while (finished < n) {
+        const double t = now_s() - t0;
+        while (next < n && req[next].arrival_s <= t) queue[qt++] = next++;
+        /* admission, first come first served */
+        while (qh < qt) {
+            semsks_request *r = &req[queue[qh]];
+            int slot = -1;
+            semsks_status a = try_admit(e, r, &slot);
+            if (a == SEMSKS_ENOHBM || a == SEMSKS_ENOSLOT) {
+                if (nact == 0) {              /* cannot fit even into an idle engine */
+                    r->status = a;
+                    r->done_s = now_s() - t0;
+                    ++qh; ++finished;
+                    continue;
+                }
+                break;
+            }
+            if (a != SEMSKS_OK) {
+                r->status = a;
+                r->done_s = now_s() - t0;
+                ++qh; ++finished;
+                continue;
+            }
+            r->slot = slot;
+            r->admit_s = now_s() - t0;
+            active[nact++] = queue[qh++];
+            if (nact > max_active) max_active = nact;
+        }
+        if (nact == 0) {
+            if (next < n) sleep_s(req[next].arrival_s - (now_s() - t0));
+            continue;
+        }
+        st = semsks_advance(e, steps_per_round, 1);
+        if (st != SEMSKS_OK) break;
+        ++rounds;
+        /* harvest the instances that reached their last level */
+        for (int i = 0; i < nact;) {
+            semsks_request *r = &req[active[i]];
+            int level = 0, done = 0;
+            st = semsks_slot_status(e, r->slot, &level, &done);
+            if (st != SEMSKS_OK) break;
+            if (!done) { ++i; continue; }
+            const semsks_problem *p = r->problem;
+            if (r->traces && p->ntrace > 0) {
+                st = semsks_get_traces(e, r->slot, 0, p->nst, r->traces);
+                if (st != SEMSKS_OK) break;
+            }
+            r->done_s = now_s() - t0;
+            point_steps += (double)(p->nex * p->hpn + 1) * (double)(p->nez * p->hpn + 1) * (double)(p->nst - 1);
+            st = semsks_retire(e, r->slot);
+            if (st != SEMSKS_OK) break;
+            active[i] = active[--nact];
+            ++finished;
+        }
+        if (st != SEMSKS_OK) break;
+    }
+    if (stats) {
+        stats->wall_s = now_s() - t0;
+        stats->point_steps = point_steps;
+        stats->completed = 0;
+        for (int i = 0; i < n; ++i) stats->completed += req[i].status == SEMSKS_OK && req[i].done_s >= 0;
+        stats->max_active = max_active;
+        stats->rounds = rounds;
+    }
+    /* on a device error, whatever is still admitted is retired */
+    for (int i = 0; i < nact; ++i) semsks_retire(e, req[active[i]].slot);
+    free(queue);
+    free(active);
+    return st;
+}

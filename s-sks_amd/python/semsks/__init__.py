@@ -29,7 +29,7 @@ EXPORTS = [
     "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_advance",
     "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_retire", "semsks_sync",
     "semsks_last_timing", "semsks_last_timed_steps", "semsks_run_problem", "semsks_stiff",
-    "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave",
+    "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve",
 ]
 
 
@@ -49,6 +49,17 @@ class Problem(C.Structure):
         ("nt", C.c_int), ("dp", C.c_float),
         ("psv", C.POINTER(C.c_float)), ("sh", C.POINTER(C.c_float)),
     ]
+
+
+class Request(C.Structure):
+    _fields_ = [("problem", C.POINTER(Problem)), ("arrival_s", C.c_double), ("use_planewave", C.c_int),
+                ("planewave", C.c_double * 4), ("traces", C.POINTER(C.c_float)),
+                ("admit_s", C.c_double), ("done_s", C.c_double), ("slot", C.c_int), ("status", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("point_steps", C.c_double), ("completed", C.c_int),
+                ("max_active", C.c_int), ("rounds", C.c_int)]
 
 
 class EngineOpts(C.Structure):
@@ -96,6 +107,7 @@ def lib() -> C.CDLL:
     L.semsks_get_boundary_series.argtypes = [vp, C.c_int, C.c_int, vp, vp]
     L.semsks_planewave.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float,
                                    C.c_double, C.c_double, C.c_double, C.c_double, vp, vp]
+    L.semsks_serve.argtypes = [vp, C.POINTER(Request), C.c_int, C.c_int, C.POINTER(ServeStats)]
     _lib = L
     return L
 
@@ -243,6 +255,39 @@ class Engine:
             return psv, sh
         _check(lib().semsks_planewave(self._h, nex, nez, hpn, neh, eh, nt, dp, p, beta, f0, psi, None, None))
         return None
+
+    def serve(self, problems, arrivals, planewaves=None, steps_per_round: int = 16, want_traces: bool = True):
+        """Continuous batching (semsks_serve): problems[i] arrives arrivals[i]
+        seconds after the start; planewaves[i] = (p, beta, f0, psi) generates
+        its boundary records on the device.  Returns (per-request dicts, stats)."""
+        n = len(problems)
+        order = sorted(range(n), key=lambda i: arrivals[i])
+        reqs = (Request * n)()
+        keep = []
+        for j, i in enumerate(order):
+            p = problems[i]
+            r = reqs[j]
+            r.problem = C.pointer(p)
+            r.arrival_s = float(arrivals[i])
+            if planewaves is not None and planewaves[i] is not None:
+                r.use_planewave = 1
+                for k in range(4):
+                    r.planewave[k] = float(planewaves[i][k])
+            if want_traces and p.ntrace > 0:
+                tr = np.zeros((p.nst, 3, p.ntrace), np.float32)
+                keep.append(tr)
+                r.traces = tr.ctypes.data_as(C.POINTER(C.c_float))
+            else:
+                keep.append(None)
+        st = ServeStats()
+        _check(lib().semsks_serve(self._h, reqs, n, steps_per_round, C.byref(st)))
+        out = [None] * n
+        for j, i in enumerate(order):
+            r = reqs[j]
+            out[i] = dict(arrival=r.arrival_s, admit=r.admit_s, done=r.done_s, status=r.status, slot=r.slot,
+                          latency=r.done_s - r.arrival_s, traces=keep[j])
+        return out, dict(wall_s=st.wall_s, point_steps=st.point_steps, completed=st.completed,
+                         max_active=st.max_active, rounds=st.rounds)
 
     def timing(self):
         t = (C.c_double * 2)()

@@ -76,12 +76,12 @@ def c3_waves():
     return [(p, baz) for p in (0.04, 0.05, 0.06, 0.07) for baz in np.arange(16) * 22.5]
 
 
-def c4_cases(n: int = 96, seed: int = SEED, nst_range=(2000, 12000)):
+def c4_cases(n: int = 96, seed: int = SEED, nst_range=(2000, 12000), nex_range=(200, 1600), nez_range=(100, 400)):
     rng = np.random.default_rng(seed + 4)
     out = []
     for k in range(n):
-        nex = int(rng.integers(200, 1601))
-        nez = int(rng.integers(100, 401))
+        nex = int(rng.integers(nex_range[0], nex_range[1] + 1))
+        nez = int(rng.integers(nez_range[0], nez_range[1] + 1))
         nl = int(rng.integers(1, 5))
         total = nez * 2.0
         cuts = np.sort(rng.uniform(0.05, 0.95, size=nl - 1)) * total
@@ -105,6 +105,22 @@ def c4_cases(n: int = 96, seed: int = SEED, nst_range=(2000, 12000)):
                                nrec=50, p=p, f0=0.2, solver=0, aniso=an, psi_deg=psi, sh_zero=False,
                                seed=seed + 100 + k))
     return out
+
+
+def c5_requests(rate: float = 20.0, duration: float = 300.0, seed: int = SEED, nex_range=(200, 1600),
+                nez_range=(100, 400), nst_range=(2000, 12000)):
+    """Poisson arrivals at `rate` per second over `duration` seconds, each a
+    C4-style instance (mesh size, 1-4 layers, iso/hexagonal, p, polarisation
+    and length drawn from the seed).  Returns [(arrival_s, case)]."""
+    rng = np.random.default_rng(seed + 5)
+    t, out = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > duration:
+            break
+        out.append(t)
+    cases = c4_cases(len(out), seed=seed + 50, nst_range=nst_range, nex_range=nex_range, nez_range=nez_range)
+    return list(zip(out, cases))
 
 
 def device_problem(model: Model, nt: int) -> Problem:

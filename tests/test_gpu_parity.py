@@ -184,3 +184,37 @@ def test_tile_height_and_autotuner_leave_results_unchanged(tmp_path, monkeypatch
     runs = [run_gpu(m.problem, tile_z=tz) for tz in (3, 16, 0)]
     for r in runs[1:]:
         assert np.array_equal(r.view(np.uint32), runs[0].view(np.uint32))
+
+
+def test_serving_results_equal_alone_and_in_the_mix(tmp_path, monkeypatch):
+    """Continuous batching (semsks_serve): requests of different mesh size,
+    model and length arrive while others run; each one's traces are the
+    same, bit for bit, as when it runs alone."""
+    print("case: serving, 6 requests vs solo", flush=True)
+    specs = [dict(nex=24, nst=300, p=0.06), dict(nex=36, nst=200, p=0.05, psi_deg=40.0, sh_zero=False),
+             dict(nex=20, nst=450, p=0.07, solver=1), dict(nex=30, nst=120, p=0.045),
+             dict(nex=24, nst=260, p=0.055, psi_deg=70.0, sh_zero=False),
+             dict(nex=28, nst=180, p=0.065, interfaces=[([0.0, 60.0], [14.0, 18.0]), None, None])]
+    models = []
+    for k, kw in enumerate(specs):
+        d = make_case(tmp_path / ("r%d" % k), **kw)
+        monkeypatch.chdir(d)
+        models.append(semsks.Model(["par=run.par"]))
+    solo = [run_gpu(m.problem) for m in models]
+    arrivals = [0.0, 0.0, 0.02, 0.05, 0.05, 0.3]
+    with semsks.Engine(max_slots=3) as e:          # fewer slots than requests: some must wait
+        res, stats = e.serve([m.problem for m in models], arrivals, steps_per_round=16)
+    assert stats["completed"] == len(models)
+    for r, s in zip(res, solo):
+        assert r["status"] == 0 and r["done"] >= r["admit"] >= r["arrival"]
+        assert np.array_equal(r["traces"].view(np.uint32), s.view(np.uint32))
+    # a sample of the served requests against the FP64 oracle (ones long
+    # enough for the incident wave, peak amplitude 1, to reach the receivers)
+    for k in (0, 2):
+        monkeypatch.chdir(os.fspath(tmp_path / ("r%d" % k)))
+        ref = pyoracle.Oracle(["par=run.par"], "f64").run()
+        assert np.abs(ref).max() > 0.1
+        for c, (l2, pk) in enumerate(misfits(res[k]["traces"], ref)):
+            if np.abs(ref[:, c]).max() == 0:
+                continue
+            assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, c, l2, pk)
