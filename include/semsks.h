@@ -91,6 +91,11 @@ typedef struct semsks_problem {
     int nt;
     float dp;
     const float *psv, *sh;
+    /* exponential sponge on the edge strips (atten_wave2 / atten_taper,
+       sem2d_psvsh.c:1725-1738,1780-1823): 0 off, 1 bottom, 2 +left,
+       3 +right, 4 +top; ass = taper value at the inner end of a strip */
+    int atten;
+    float ass;
 } semsks_problem;
 
 /* ---------------------------------------------------------------------- */

@@ -416,6 +416,51 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
     }
 }
 
+/* atten_wave2 (sem2d_psvsh.c:1780-1823): after the update, u(t+dt) on the
+   edge strips of nh = neh*hpn nodes is multiplied by the taper (bottom for
+   atten >= 1, left >= 2, right >= 3, top >= 4; each node at most once).
+   The reference then rotates the tapered u(t+dt) into u, so delta becomes
+   u'(t+dt) - u(t) and the compensation of the rescaled value restarts. */
+__global__ void sponge_kernel(const InstDesc *insts, const int *levels)
+{
+    const InstDesc &D = insts[blockIdx.y];
+    if (!D.active || D.atten <= 0) return;
+    const int it = levels[blockIdx.y];
+    if (it >= D.nst - 1) return;
+    const int p = it & 1, nh = D.nh, nx = D.nx, nz = D.nz, ld = D.ld, at = D.atten;
+    const long nb = (long)nh * nx, ns = (long)(nz - nh) * nh, ntop = (long)nh * (nx - 2 * nh);
+    const long total = nb + (at > 1 ? ns : 0) + (at > 2 ? ns : 0) + (at > 3 ? ntop : 0);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        int ix, iz;
+        float f;
+        long j = i;
+        if (j < nb) {
+            iz = (int)(j / nx); ix = (int)(j - (long)iz * nx);
+            f = D.taper[nh - iz - 1];
+        } else if ((j -= nb) < ns) {
+            iz = nh + (int)(j / nh); ix = (int)(j % nh);
+            f = D.taper[nh - ix - 1];
+        } else if ((j -= ns) < ns) {
+            iz = nh + (int)(j / nh); ix = nx - nh + (int)(j % nh);
+            f = D.taper[ix - nx + nh];
+        } else {
+            j -= ns;
+            iz = nz - nh + (int)(j / (nx - 2 * nh)); ix = nh + (int)(j % (nx - 2 * nh));
+            f = D.taper[iz - nz + nh];
+        }
+        const long q = (long)iz * ld + ix;
+        float *U1 = D.u[p ^ 1] + 3 * q;
+        const float *U0 = D.u[p] + 3 * q;
+        float *S = D.st + 4 * q;
+        for (int k = 0; k < 3; ++k) {
+            const float un = f * U1[k];
+            S[k] = un - U0[k];
+            U1[k] = un;
+        }
+        S[3] = 0.f;
+    }
+}
+
 /* kirrecord_zhl: u to seis.u, -w to seis.w, v to seis.v; then advance the level */
 __global__ void record_kernel(const InstDesc *insts, int *levels)
 {
@@ -1113,6 +1158,7 @@ struct semsks_engine {
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
     unsigned hpn_mask = 0, graph_mask = 0;
+    bool any_sponge = false, graph_sponge = false;
     int graph_off[SEMSKS_MAXNP] = {0};  /* tile offsets the captured step was built with */
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> tev;      /* per-step timing events */
@@ -1158,6 +1204,8 @@ static semsks_status upload_tables(semsks_engine *e)
         h[i].active = e->slots[i].used ? 1 : 0;
     }
     HIPCHK(hipMemcpyAsync(e->d_desc, h.data(), h.size() * sizeof(InstDesc), hipMemcpyHostToDevice, e->stream));
+    e->any_sponge = false;
+    for (auto &s : e->slots) e->any_sponge |= s.used && s.d.atten > 0;
     /* tiles, grouped by hpn, mixed (per-node material) tiles first, slots
        interleaved per spatial tile so that concurrent waves touch the same
        region of a shared mesh */
@@ -1255,6 +1303,7 @@ static semsks_status launch_step(semsks_engine *e, hipEvent_t ta, hipEvent_t tb)
         if (e->hpn_mask & (1u << n))
             launch_tiles<false>(e, n, e->d_desc, e->d_levels, e->d_tiles + e->tile_off[n], e->d_ntiles + n);
     if (tb) HIPCHK(hipEventRecord(tb, e->stream));
+    if (e->any_sponge) hipLaunchKernelGGL(sponge_kernel, dim3(64, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
     hipLaunchKernelGGL(record_kernel, dim3(nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
     HIPCHK(hipGetLastError());
     return SEMSKS_OK;
@@ -1279,6 +1328,7 @@ static semsks_status build_graph(semsks_engine *e, int steps)
     HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
     e->graph_steps = steps;
     e->graph_mask = e->hpn_mask;
+    e->graph_sponge = e->any_sponge;
     memcpy(e->graph_off, e->tile_off, sizeof e->graph_off);
     return SEMSKS_OK;
 }
@@ -1445,6 +1495,23 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         d.u[0] = st_all;
         d.u[1] = st_all + 3 * nodes;
         d.st = st_all + 6 * nodes;
+    }
+    if (p->atten > 0) {   /* atten_taper (sem2d_psvsh.c:1725-1738), double then float */
+        const int nh = p->neh * p->hpn;
+        if (nh < 1 || s.nx < 2 * nh || s.nz < 2 * nh) {
+            st = semsks_set_error(SEMSKS_EINVAL, "atten: strips of %d nodes do not fit the mesh", nh);
+            goto fail;
+        }
+        std::vector<float> tp(nh);
+        const double alpha = 1.0 / nh * log10((double)p->ass) / log10(2.73);
+        for (int i = 0; i < nh; ++i) tp[i] = (float)exp(alpha * i);
+        float *dt = nullptr;
+        ACHK(dalloc(s, &dt, (size_t)nh));
+        ACHK(hipMemcpyAsync(dt, tp.data(), nh * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        ACHK(hipStreamSynchronize(e->stream));
+        d.taper = dt;
+        d.atten = p->atten;
+        d.nh = nh;
     }
     if (share >= 0) {
         const InstDesc &o = e->slots[share].d;
@@ -1653,7 +1720,7 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
         const int G = 16;
         if (nsteps - done >= G) {
             if (!e->gexec || e->graph_steps != G || e->graph_mask != e->hpn_mask ||
-                memcmp(e->graph_off, e->tile_off, sizeof e->graph_off) != 0) {
+                memcmp(e->graph_off, e->tile_off, sizeof e->graph_off) != 0 || e->graph_sponge != e->any_sponge) {
                 semsks_status st = build_graph(e, G);
                 if (st) return st;
             }

@@ -504,12 +504,10 @@ semsks_status semsks_load_par(int argc, char **argv, semsks_model **out)
     get_i(&P, "kahan", &kahan);
     snprintf(M->outdir, sizeof M->outdir, ".");
     get_s(&P, "outdir", M->outdir, sizeof M->outdir);
-    {
-        int atten = 0;
-        get_i(&P, "atten", &atten);
-        if (atten > 0 && st == SEMSKS_OK)
-            st = semsks_set_error(SEMSKS_EINVAL, "atten>0 (sponge layer, off by default) is not supported");
-    }
+    int atten = 0;
+    float ass = 0.93f;
+    get_i(&P, "atten", &atten);
+    get_f(&P, "ass", &ass);
     int stop = pl_get(&P.L, "STOP") != NULL;
     if (st == SEMSKS_OK && P.missing)
         st = semsks_set_error(SEMSKS_EPAR, "must specify value for '%s'", P.missing);
@@ -527,6 +525,7 @@ semsks_status semsks_load_par(int argc, char **argv, semsks_model **out)
     if (st != SEMSKS_OK) { semsks_model_free(M); return st; }
     p->nex = nex; p->nez = nez; p->hpn = hpn; p->neh = neh; p->eh = eh; p->sdt = sdt; p->nst = nst;
     p->solver = solver; p->absorb = absorb; p->flags = kahan ? 0 : SEMSKS_F_NO_KAHAN;
+    p->atten = atten; p->ass = ass;
     M->nx = nex * hpn + 1;
     M->nz = nez * hpn + 1;
     if (nez <= neh) st = semsks_set_error(SEMSKS_EINVAL, "nez=%d must exceed neh=%d", nez, neh);

@@ -42,6 +42,8 @@ struct InstDesc {
     const float *edge_a, *edge_b;   /* [n_edge][3] */
     const float *G, *Gv;            /* [nt_sem][2(nx+nz)], [nt_sem][nx+nz] */
     const int *rec;                 /* node offsets iz*ld+ix */
+    const float *taper;             /* sponge factors [nh] (atten > 0) */
+    int atten, nh;
     float *traces;
     /* element operators (FP32 from FP64):
        Dg[a][t] = l'_t(xi_a) * 2/eh      (gradient, dlagP_JeI)
@@ -50,7 +52,7 @@ struct InstDesc {
     float Dg[SEMSKS_MAXNP][SEMSKS_MAXNP];
     float Dw[SEMSKS_MAXNP][SEMSKS_MAXNP];
     float cw[SEMSKS_MAXNP];
-    int pad[5];
+    int pad[3];
 };
 
 #if defined(__HIPCC__)

@@ -48,6 +48,7 @@ class Problem(C.Structure):
         ("ntrace", C.c_int), ("rec_ix", C.POINTER(C.c_int)), ("rec_iz", C.POINTER(C.c_int)),
         ("nt", C.c_int), ("dp", C.c_float),
         ("psv", C.POINTER(C.c_float)), ("sh", C.POINTER(C.c_float)),
+        ("atten", C.c_int), ("ass", C.c_float),
     ]
 
 

@@ -218,3 +218,21 @@ def test_serving_results_equal_alone_and_in_the_mix(tmp_path, monkeypatch):
             if np.abs(ref[:, c]).max() == 0:
                 continue
             assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, c, l2, pk)
+
+
+@pytest.mark.parametrize("atten", [1, 4])
+def test_sponge_strips_vs_fp64(small, atten):
+    """atten_wave2's exponential taper on the edge strips (bottom only, and
+    all four edges) against the FP64 oracle."""
+    print("case: small, atten", atten, flush=True)
+    args = ["par=run.par", "atten=%d" % atten, "ass=0.9"]
+    m = semsks.Model(args)
+    assert m.problem.atten == atten
+    tr = run_gpu(m.problem)
+    ref = pyoracle.Oracle(args, "f64").run()
+    base = pyoracle.Oracle(["par=run.par"], "f64").run()
+    assert np.abs(ref - base).max() > 1e-3 * np.abs(base).max()       # the sponge changes the traces
+    for k, (l2, pk) in enumerate(misfits(tr, ref)):
+        if np.abs(ref[:, k]).max() == 0:
+            continue
+        assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, l2, pk)
