@@ -298,3 +298,79 @@ def test_faithful_fp32_tracks_fp64(tmp_path):
     for k in range(3):
         scale = max(np.linalg.norm(t64[:, k]), 1e-3 * ref)
         assert np.linalg.norm(t32[:, k] - t64[:, k]) <= 1e-5 * scale
+
+
+# ------------------------------------------------------- absorbing edges
+@pytest.mark.parametrize("pol", ["P", "S"])
+def test_lysmer_edge_absorbs_normal_incidence(tmp_path, pol):
+    """A plane pulse running down onto the bottom edge: with the Lysmer
+    dashpots (Absorb_Matrix, Absorbs) it leaves the box, without them
+    (absorb=0, free edge) it comes back whole.  A dashpot off by a factor
+    2, or the P and S impedances swapped, reflects 1/3 or ~27%."""
+    vp, vs, rho = 6.0, 3.5, 2.7
+    found = {}
+    for absorb in (1, 0):
+        # 160 km wide, so that what the side edges do to a plane pulse does
+        # not reach the middle columns in the time measured
+        c = inputs.Case(nex=80, eh=2.0, layers_thk=(10.0, 10.0, 20.002), vp=(vp,) * 3, vs=(vs,) * 3,
+                        rho=(rho,) * 3, jitter=0.0, nst=3, nrec=2, absorb=absorb)
+        d = os.fspath(tmp_path / ("a%d" % absorb))
+        c.write(d)
+        o = pyoracle.Oracle(["par=run.par"], "f64", cwd=d)
+        zs = inputs.node_coords(o.nez, o.eh, o.hpn)[:, None] + np.zeros((1, o.nx))
+        v = vp if pol == "P" else vs
+        zc, width = 20.0, 4.0
+        shape = lambda z: np.exp(-((z - zc) / width) ** 2)
+        k = 2 if pol == "P" else 0                 # normal (w) or tangential (u) motion
+        u0 = np.zeros((3, o.nz, o.nx)); um = np.zeros_like(u0)
+        u0[k] = shape(zs)                          # u(z, t) = shape(z + v t): running down
+        um[k] = shape(zs - v * o.sdt)
+        nsteps = int(round((zc + 12.0) / v / o.sdt))   # pulse centre to the edge and 12 km back
+        for _ in range(nsteps):
+            u1 = o.update(u0, um, o.stiff(u0))
+            um, u0 = u0, u1
+        mid = slice(2 * o.nx // 5, 3 * o.nx // 5)
+        found[absorb] = np.abs(u0[k][:, mid]).max()
+    assert found[0] > 0.8          # free edge: full reflection
+    assert found[1] < 0.05         # Lysmer edge: the pulse has left
+
+
+# ----------------------------------------------------- boundary injection
+@pytest.mark.parametrize("solver,psi", [(0, 0.0), (0, 90.0), (1, 0.0), (2, 0.0)])
+def test_injected_vertical_plane_wave_amplitude(tmp_path, solver, psi):
+    """A vertically incident S pulse g(t) (peak 1) injected on the
+    interface row, looked at in the interior before it reaches the surface.
+
+    GRT_Interface3 (solver 2) replaces the interface value by the scattered
+    increment plus g, so the interior carries g itself (scale 1).
+
+    GRT_Interface4 (solvers 0, 1) adds g(t) - g(t-dt) to the interface nodes
+    every step.  In the central-difference update a displacement kick Dg on a
+    node adds Dg/dt to its velocity, i.e. the impulse (mass per area) Dg/dt;
+    the interface row's lumped mass per unit area is rho*w_0*eh (GLL end
+    weight w_0 from the elements above and below, Jacobian eh/2 each), and a
+    plane impulse in a medium of impedance rho*beta radiates half of it each
+    way, so the up-going wave is g * w_0*eh / (2*beta*dt)."""
+    beta = 3.5
+    c = inputs.Case(nex=80, eh=2.0, layers_thk=(20.0, 20.0, 20.002), vp=(6.0,) * 3, vs=(beta,) * 3,
+                    rho=(2.7,) * 3, jitter=0.0, p=0.0, f0=0.2, psi_deg=psi, sh_zero=(psi == 0.0),
+                    solver=solver, nrec=3, nst=3)
+    D = c.derived()
+    lead = 1.2 / c.f0
+    c.nst = int(round((lead + 30.0 / beta) / D["sdt"])) + 1      # pulse centre at z = 30 km
+    d = os.fspath(tmp_path)
+    c.write(d)
+    o = pyoracle.Oracle(["par=run.par"], "f64", cwd=d)
+    _, f = o.run(want_field=True)
+    t = (c.nst - 1) * o.sdt
+    zs = inputs.node_coords(o.nez, o.eh, o.hpn)
+    comp = 0 if psi == 0.0 else 1
+    col = f[comp][:, o.nx // 2]
+    inc = inputs.ricker(t - lead - zs / beta, c.f0)
+    sel = (zs > 8.0) & (zs < 52.0)
+    scale = float(np.dot(col[sel], inc[sel]) / np.dot(inc[sel], inc[sel]))
+    x, w, _ = pyoracle.gll(o.hpn, "f64")
+    expect = 1.0 if solver == 2 else w[0] * o.eh / (2.0 * beta * o.sdt)
+    assert abs(scale / expect - 1.0) < 0.01, (scale, expect)
+    assert np.abs(col[sel] - scale * inc[sel]).max() < 0.03 * abs(scale)
+    assert np.abs(f[1 - comp if comp < 2 else 0][:, o.nx // 2]).max() < 1e-9 * abs(scale)
