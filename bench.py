@@ -260,6 +260,27 @@ def main():
                "kernel_ms": k2, "bytes_per_point_step_alg": B}
         cpu_case = lambda: configs.c2_case(nst=10)
         traffic = profiled_traffic("step_kernel_plain_2000x500.md")
+    elif args.config == "c4":
+        # uneven batch: 96 instances, 200x100 .. 1600x400 elements, 1-4 layers
+        # iso/hexagonal, all stepped by one persistent launch per step
+        cases = configs.c4_cases(96)
+        t0 = time.time()
+        built = [configs.Built(c) for c in cases]
+        log("c4: built %d meshes in %.1fs (%.3g GLL points)" % (len(built), time.time() - t0,
+                                                                  sum(b.nglob() for b in built)))
+        waves = [(b, b.case.p, b.case.psi_deg) for b in built]
+        r4 = run_engine(waves, args.steps, args.warmup, False, tile_z=args.tile_z, extra_steps=args.kernel_steps)
+        k4, _ = kernel_time(r4["engine"], args.kernel_steps)
+        r4["engine"].close()
+        value, ms, npts, steps = r4["throughput"], r4["ms_per_step"], r4["npts"], args.steps
+        n_c = [15 if any(a is not None for a in (b.case.aniso or [])) else 9 for b in built]
+        B = sum((36.0 + 4.0 * (nc + 1)) * b.nglob() for nc, b in zip(n_c, built)) / npts
+        achieved = B * npts / (k4 * 1e-3) / 1e9
+        traffic = None
+        cfg = {"workload": "uneven batch: 96 instances, 200x100..1600x400 elements, 1-4 layers, iso or "
+                           "hexagonal up to 6%, one persistent kernel per step", "instances": 96,
+               "nglob_total": int(npts), "kernel_ms": k4, "bytes_per_point_step_alg": B}
+        cpu_case = lambda: configs.c2_case(nst=10)
     elif args.config == "c5":
         # the full serving configuration: 20/s for 300 s of C4-size requests
         sv = run_serving(20.0, 300.0, (200, 1600), (100, 400), (2000, 12000))
