@@ -236,3 +236,30 @@ def test_sponge_strips_vs_fp64(small, atten):
         if np.abs(ref[:, k]).max() == 0:
             continue
         assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, l2, pk)
+
+
+@pytest.mark.parametrize("hpn", [3, 5, 6])
+def test_other_polynomial_degrees_vs_fp64(tmp_path, monkeypatch, hpn):
+    """hpn = 3, 5, 6 (NGLL 4, 6, 7; lagrange.c:327-330 allows 3..6): K u and
+    the seismograms against the FP64 oracle, over a dipping interface."""
+    print("case: small, hpn", hpn, flush=True)
+    kw = dict(psi_deg=25.0, sh_zero=False, interfaces=[([0.0, 50.0], [14.0, 17.0]), None, None])
+    T = 500 * inputs.small_case(**kw).derived()["sdt"]          # the same span of time for every degree
+    nst = int(T / inputs.small_case(hpn=hpn, **kw).derived()["sdt"])
+    d = make_case(tmp_path, hpn=hpn, nst=nst, **kw)
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    assert m.problem.hpn == hpn
+    o64 = pyoracle.Oracle(["par=run.par"], "f64")
+    u = np.random.default_rng(hpn).standard_normal((3, m.nz, m.nx)).astype(np.float32)
+    with semsks.Engine() as e:
+        s = e.admit(m.problem)
+        f = e.stiff(s, u)
+    f64 = o64.stiff(u.astype(np.float64))
+    assert np.abs(f - f64).max() / np.abs(f64).max() < 2e-6
+    tr = run_gpu(m.problem)
+    ref = o64.run()
+    for k, (l2, pk) in enumerate(misfits(tr, ref)):
+        if np.abs(ref[:, k]).max() == 0:
+            continue
+        assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, l2, pk)
