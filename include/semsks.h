@@ -122,6 +122,14 @@ void semsks_model_free(semsks_model *m);
    sem2d_psvsh.c:2116-2165). */
 semsks_status semsks_write_seis(const char *outdir, int nst, int ntrace, const float *traces);
 
+/* The uvw2sac step after the solve (semsyn/plot/uvw2sac.c:40-526): reads
+   seis.u/.v/.w and writes <k>.ce/.cn/.cz SAC files (u east, v north, w
+   vertical; every second sample, x enlarge), pm.xy and log.txt, from the
+   same parameter keys (raymodel nex nst hpn eh dp sdt zbeg idzplot xbeg
+   idxplot zs Conv_Source ...; Conv_Source=1 convolves the trapezoid source
+   of source.f:137-163).  Extension keys: indir=, outdir=. */
+semsks_status semsks_uvw2sac(int argc, char **argv);
+
 /* Human-readable text of the last error on this thread. */
 const char *semsks_last_error(void);
 

@@ -7,6 +7,24 @@ extern "C" {
 #endif
 /* records an error text for semsks_last_error() and returns st */
 semsks_status semsks_set_error(semsks_status st, const char *fmt, ...);
+
+/* getpar semantics (semsyn/src/getpar/src/getpar.c) for the library's
+   readers: open = setpar, semsks_par_* = getpar (1 if the key is present),
+   semsks_mstpar_* = mstpar, close = endpar (EPAR for a missing mstpar key,
+   ESTOP for STOP in the input) */
+typedef struct semsks_pars semsks_pars;
+semsks_pars *semsks_pars_open(int argc, char **argv);
+int semsks_par_i(semsks_pars *q, const char *k, int *v);
+int semsks_par_f(semsks_pars *q, const char *k, float *v);
+int semsks_par_s(semsks_pars *q, const char *k, char *v, size_t cap);
+void semsks_mstpar_i(semsks_pars *q, const char *k, int *v);
+void semsks_mstpar_f(semsks_pars *q, const char *k, float *v);
+void semsks_mstpar_s(semsks_pars *q, const char *k, char *v, size_t cap);
+semsks_status semsks_pars_close(semsks_pars *q);
+
+/* nez and eh of the SEM mesh from the raymodel thickness (readmodel.c:43-109,
+   sem2d_psvsh.c:591-611) */
+semsks_status semsks_raymodel_mesh(const char *fn, int flat, int nfil, float eh_in, int *nez, float *eh);
 #ifdef __cplusplus
 }
 #endif

@@ -446,11 +446,9 @@ static semsks_status read_records(const char *fn, int nt, int width, float **out
     return SEMSKS_OK;
 }
 
-semsks_status semsks_load_par(int argc, char **argv, semsks_model **out)
+/* setpar (getpar.c:137-260): name=value arguments, par= files */
+static void pars_args(pars *P, int argc, char **argv)
 {
-    pars P;
-    memset(&P, 0, sizeof P);
-    *out = NULL;
     for (int i = 1; i < argc; ++i) {
         const char *a = argv[i];
         char name[256], val[8192];
@@ -459,12 +457,56 @@ semsks_status semsks_load_par(int argc, char **argv, semsks_model **out)
         while (*a && *a != '=') { if (nl + 1 < sizeof name) name[nl++] = *a; ++a; }
         name[nl] = 0;
         if (*a == '=') ++a;
-        scan_value(&P.L, a, val, sizeof val, 0);
-        if (name[0] == '-') pl_add(&P.L, "SWITCH", name + 1);
-        else pl_add(&P.L, name, val);
-        if (!strcmp(name, "par")) pl_file(&P.L, val, 1);
+        scan_value(&P->L, a, val, sizeof val, 0);
+        if (name[0] == '-') pl_add(&P->L, "SWITCH", name + 1);
+        else pl_add(&P->L, name, val);
+        if (!strcmp(name, "par")) pl_file(&P->L, val, 1);
         if (!strcmp(name, "ENDPAR")) break;
     }
+}
+
+/* the same parameter machinery for the library's other readers */
+struct semsks_pars { pars P; };
+
+semsks_pars *semsks_pars_open(int argc, char **argv)
+{
+    semsks_pars *q = (semsks_pars *)calloc(1, sizeof *q);
+    if (q) pars_args(&q->P, argc, argv);
+    return q;
+}
+int semsks_par_i(semsks_pars *q, const char *k, int *v) { return get_i(&q->P, k, v); }
+int semsks_par_f(semsks_pars *q, const char *k, float *v) { return get_f(&q->P, k, v); }
+int semsks_par_s(semsks_pars *q, const char *k, char *v, size_t cap) { return get_s(&q->P, k, v, cap); }
+void semsks_mstpar_i(semsks_pars *q, const char *k, int *v) { req_i(&q->P, k, v); }
+void semsks_mstpar_f(semsks_pars *q, const char *k, float *v) { req_f(&q->P, k, v); }
+void semsks_mstpar_s(semsks_pars *q, const char *k, char *v, size_t cap) { req_s(&q->P, k, v, cap); }
+
+semsks_status semsks_pars_close(semsks_pars *q)
+{
+    semsks_status st = SEMSKS_OK;
+    if (q->P.L.st) st = q->P.L.st;
+    else if (q->P.missing) st = semsks_set_error(SEMSKS_EPAR, "must specify value for '%s'", q->P.missing);
+    else if (pl_get(&q->P.L, "STOP")) st = semsks_set_error(SEMSKS_ESTOP, "stop due to STOP in input");
+    pl_free(&q->P.L);
+    free(q);
+    return st;
+}
+
+semsks_status semsks_raymodel_mesh(const char *fn, int flat, int nfil, float eh_in, int *nez, float *eh)
+{
+    int nrec = 0, jo = 0;
+    float *cc = NULL, *ss = NULL, *dd = NULL, *tth = NULL, thk = 0;
+    semsks_status st = read_raymodel(fn, flat, nfil, eh_in, nez, eh, &nrec, &jo, &cc, &ss, &dd, &tth, &thk);
+    free(cc); free(ss); free(dd); free(tth);
+    return st;
+}
+
+semsks_status semsks_load_par(int argc, char **argv, semsks_model **out)
+{
+    pars P;
+    memset(&P, 0, sizeof P);
+    *out = NULL;
+    pars_args(&P, argc, argv);
     semsks_model *M = (semsks_model *)calloc(1, sizeof *M);
     semsks_problem *p = &M->prob;
     int flat = 1, nfil = 1, readpars = 1, get_rec = 0, xtrace = 20, idxplot = 20, msource = 0, nex = 0;

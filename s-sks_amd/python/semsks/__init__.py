@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.normpath(os.path.join(_HERE, "..", ".."))          # s-sks_amd/
 LIB_PATH = os.path.join(PKG_ROOT, "build", "libsemsks.so")
 CLI_PATH = os.path.join(PKG_ROOT, "build", "semsks")
+SAC_CLI_PATH = os.path.join(PKG_ROOT, "build", "semsks_uvw2sac")
 
 MAT_STRIDE = 16
 SOLVER_PSVSH, SOLVER_PSV, SOLVER_PSV_MP = 0, 1, 2
@@ -29,7 +30,7 @@ EXPORTS = [
     "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_advance",
     "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_retire", "semsks_sync",
     "semsks_last_timing", "semsks_last_timed_steps", "semsks_run_problem", "semsks_stiff",
-    "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve",
+    "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve", "semsks_uvw2sac",
 ]
 
 
