@@ -263,3 +263,30 @@ def test_other_polynomial_degrees_vs_fp64(tmp_path, monkeypatch, hpn):
         if np.abs(ref[:, k]).max() == 0:
             continue
         assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, l2, pk)
+
+
+def test_splitting_of_fp32_traces_matches_fp64(tmp_path, monkeypatch):
+    """The FP32 traces keep the anisotropic physics: the Silver & Chan
+    transverse-energy map over (fast azimuth, delay) of the GPU seismogram
+    matches the FP64 oracle's, and so does its minimum."""
+    from semsks import aniso, splitting
+    print("case: splitting, tilted hexagonal 8% bottom layer", flush=True)
+    C = aniso.hexagonal_tilted(vp=7.8, vs=4.4, rho=3.3, strength=0.08, dip_deg=20.0, azimuth_deg=40.0)
+    d = make_case(tmp_path, nex=30, nst=900, aniso=[None, None, C], psi_deg=0.0, sh_zero=False, p=0.04, f0=0.6)
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    gpu = run_gpu(m.problem)
+    o = pyoracle.Oracle(["par=run.par"], "f64")
+    ref = o.run()
+    r = m.problem.ntrace // 2
+    dt = o.sdt
+    res = []
+    for tr in (gpu, ref):
+        u, v = tr[:, 0, r].astype(np.float64), tr[:, 2, r].astype(np.float64)
+        ipk = int(np.argmax(np.hypot(u, v)))
+        win = (max(0, ipk - int(3 / dt)), min(len(u), ipk + int(5 / dt)))
+        res.append(splitting.silver_chan(v, u, dt, 270.0, win, delays=np.arange(0, 1.0001, 0.02)))
+    a, b = res
+    assert b["delay"] > 0                                       # the layer splits the wave
+    assert np.abs(a["E"] - b["E"]).max() <= 1e-3 * b["E"].max()
+    assert abs(a["phi"] - b["phi"]) <= 1.0 and abs(a["delay"] - b["delay"]) <= 0.02 + 1e-9
