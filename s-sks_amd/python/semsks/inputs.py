@@ -17,10 +17,12 @@ Writes the files the SEM solver reads (`sem2d_psvsh par=run.par`):
   (semsyn/src/sem2d_psvsh.c:347-358).
 * ``run.par``   -- getpar ``name=value`` lines.
 
-The incident wavefield is a closed-form plane wave in the bottom layer,
-evaluated at the true GLL coordinates of the interface nodes (row ``nbl`` and
-column ``nbl``, ``nbl = neh*hpn``).  It stands in for the GRT producer, which
-needs Fortran and is not built.
+The boundary wavefield is evaluated at the true GLL coordinates of the
+interface nodes (row ``nbl`` and column ``nbl``, ``nbl = neh*hpn``), either as
+the closed-form incident plane wave of the bottom layer (``boundary="plane"``)
+or as the total field of the flat 1-D model with its free surface and
+interface reverberations (``boundary="layered"``, semsks.layered).  Both stand
+in for the GRT producer, which needs Fortran and is not built.
 """
 from __future__ import annotations
 
@@ -272,6 +274,8 @@ class Case:
     aniso: list | None = None        # optional per-layer (constants dict, dip6)
     extra: dict = field(default_factory=dict)
     nez: int | None = None           # if set, the bottom layer is fitted to give exactly nez elements
+    boundary: str = "plane"          # "plane": the incident wave only; "layered": total field of the
+                                     # flat 1-D model with its free surface (semsks.layered)
 
     def __post_init__(self):
         if self.nez is not None:
@@ -330,8 +334,14 @@ class Case:
         pw = PlaneWave(p=self.p, beta=vs[-1], f0=self.f0, psi_deg=self.psi_deg)
         if write_records:
             # record times use the float32 dp the solver reads
-            psv, sh = greenfile_records(pw, self.nex, D["nez"], D["eh"], self.hpn, self.neh, nt,
-                                        float(F32(sdt)))
+            if self.boundary == "layered":
+                from .layered import layered_records
+                psv, sh = layered_records(list(self.layers_thk), list(vp) + [vp[-1]], list(vs) + [vs[-1]],
+                                          list(rho) + [rho[-1]], self.p, self.f0, self.psi_deg, self.nex,
+                                          D["nez"], D["eh"], self.hpn, self.neh, nt, float(F32(sdt)))
+            else:
+                psv, sh = greenfile_records(pw, self.nex, D["nez"], D["eh"], self.hpn, self.neh, nt,
+                                            float(F32(sdt)))
             if self.sh_zero:
                 sh[:] = 0
         else:   # placeholders: the caller supplies the records another way

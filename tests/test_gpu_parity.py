@@ -290,3 +290,18 @@ def test_splitting_of_fp32_traces_matches_fp64(tmp_path, monkeypatch):
     assert b["delay"] > 0                                       # the layer splits the wave
     assert np.abs(a["E"] - b["E"]).max() <= 1e-3 * b["E"].max()
     assert abs(a["phi"] - b["phi"]) <= 1.0 and abs(a["delay"] - b["delay"]) <= 0.02 + 1e-9
+
+
+def test_layered_boundary_records_vs_fp64(tmp_path, monkeypatch):
+    """Boundary records from the layered-medium producer (free-surface and
+    interface reverberations in the greenfile) through the GPU solver,
+    against the FP64 oracle on the same files."""
+    print("case: small, layered boundary records", flush=True)
+    d = make_case(tmp_path, boundary="layered", psi_deg=35.0, sh_zero=False, p=0.05)
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    tr = run_gpu(m.problem)
+    ref = pyoracle.Oracle(["par=run.par"], "f64").run()
+    assert np.abs(ref).max() > 0.1
+    for k, (l2, pk) in enumerate(misfits(tr, ref)):
+        assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, l2, pk)
