@@ -173,7 +173,9 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync);
 /* Current time level of a slot (-1 if empty) and whether it finished. */
 semsks_status semsks_slot_status(semsks_engine *e, int slot, int *level, int *done);
 
-/* Copies traces [n][3][ntrace] of levels it0..it0+n-1 of a slot to host. */
+/* Copies traces [n][3][ntrace] of levels it0..it0+n-1 of a slot to host;
+   returns SEMSKS_ENONFINITE (traces still copied) if any value is NaN or
+   infinite, i.e. the run became unstable. */
 semsks_status semsks_get_traces(semsks_engine *e, int slot, int it0, int n, float *out);
 
 /* Copies the current displacement field of a slot to host as three SoA

@@ -1796,6 +1796,12 @@ semsks_status semsks_get_traces(semsks_engine *e, int k, int it0, int n, float *
     const size_t w = (size_t)3 * s.meta.ntrace;
     if (w && n) HIPCHK(hipMemcpyAsync(out, s.d.traces + (size_t)it0 * w, (size_t)n * w * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    /* a run that became unstable (the reference lets NaN/Inf propagate
+       into seis.*) is reported; the traces are still copied */
+    for (size_t i = 0; i < (size_t)n * w; ++i)
+        if (!isfinite(out[i]))
+            return semsks_set_error(SEMSKS_ENONFINITE, "slot %d: non-finite trace value at level %zu", k,
+                                    (size_t)it0 + i / w);
     return SEMSKS_OK;
 }
 

@@ -114,6 +114,7 @@ semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int ste
             const semsks_problem *p = r->problem;
             if (r->traces && p->ntrace > 0) {
                 st = semsks_get_traces(e, r->slot, 0, p->nst, r->traces);
+                if (st == SEMSKS_ENONFINITE) { r->status = st; st = SEMSKS_OK; }   /* an unstable request */
                 if (st != SEMSKS_OK) break;
             }
             r->done_s = now_s() - t0;

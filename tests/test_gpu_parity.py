@@ -305,3 +305,21 @@ def test_layered_boundary_records_vs_fp64(tmp_path, monkeypatch):
     assert np.abs(ref).max() > 0.1
     for k, (l2, pk) in enumerate(misfits(tr, ref)):
         assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, l2, pk)
+
+
+def test_unstable_run_is_reported(tmp_path, monkeypatch):
+    """A time step far above the stability limit blows the field up; the
+    traces come back with SEMSKS_ENONFINITE instead of silently."""
+    print("case: small, sdt x 4 (unstable)", flush=True)
+    c = inputs.small_case(nst=400)
+    c.sdt = 4.0 * c.derived()["sdt"]
+    d = os.fspath(tmp_path)
+    c.write(d)
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    with semsks.Engine() as e:
+        s = e.admit(m.problem)
+        e.advance(m.problem.nst - 1)
+        with pytest.raises(semsks.SemsksError) as ei:
+            e.traces(s)
+        assert ei.value.code == 4
