@@ -14,7 +14,9 @@
  *   interior : delta_n = delta_{n-1} - (dt^2/M) f(u_n)
  *   edges    : delta_n = ((M-A) delta_{n-1} - dt^2 f) / (M+A)
  * which is algebraically the reference's u+ = -(dt^2/M) f + 2u - u- and its
- * absorbing rows; u_{n+1} = u_n + delta_n is a Kahan-compensated sum.
+ * absorbing rows.  delta_{n-1} comes from the compensated values of u_n and
+ * u_{n-1} (each u + packed compensation), and u_{n+1} = u_n + delta_n is a
+ * Kahan-compensated sum.
  *
  * Element force (row-per-lane layout): a 64-lane wave owns a tile of
  * NE-1 = 64/(N+1)-1 element columns (11 for N = 4) x TZ element rows.  The
@@ -30,9 +32,10 @@
  * node is written by exactly one lane, which applies the update in the same
  * pass while the row of u is still in registers.
  *
- * State in HBM: u(t) and u(t+dt) as xyz-interleaved rows (12 bytes per
- * node), and one 16-byte record per node holding delta (xyz) and the packed
- * Kahan compensation of u.  Rows are padded to a multiple of 64 nodes.
+ * State in HBM: three time levels of u rotating (level it in buffer it % 3),
+ * one 16-byte record per node {x, y, z, packed Kahan compensation}.  A step
+ * reads level it (with halos) and level it-1 (owned nodes) and writes level
+ * it+1.  Rows are padded to a multiple of 64 nodes.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -295,7 +298,7 @@ __global__ void series_kernel(const float *rec, int width, int nrec, const Sched
 /* ========================================================================= */
 
 /* GRT_Interface at it == 0 (sem2d_psvsh.c:1953-2005): u on the interface row
-   and column from g(0), u(-dt) = 0, so delta = u */
+   and column from g(0); u(-dt) = 0 (the cleared buffer of level -1) */
 __global__ void init_state_kernel(InstDesc D)
 {
     const int nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld;
@@ -316,8 +319,7 @@ __global__ void init_state_kernel(InstDesc D)
             q = (long)iz * ld + nbl;
             u = g[2 * nx + iz]; w = -g[2 * nx + nz + iz]; if (sh) v = gv[nx + iz];
         }
-        D.u[0][3 * q] = u; D.u[0][3 * q + 1] = v; D.u[0][3 * q + 2] = w;
-        D.st[4 * q] = u; D.st[4 * q + 1] = v; D.st[4 * q + 2] = w;
+        D.u[0][4 * q] = u; D.u[0][4 * q + 1] = v; D.u[0][4 * q + 2] = w;
     }
 }
 
@@ -353,8 +355,8 @@ __device__ __forceinline__ uint32_t c_pack(float c, float u)
 }
 
 /* GRT_Interface4: u += g(t) - g(t-dt) (w sign flipped); GRT_Interface3:
-   u += -u(t-dt) + g(t).  Both change u(t) in place before the force, and the
-   reference then copies the changed u(t) into u(t-dt), so delta moves with u. */
+   u += -u(t-dt) + g(t).  Both change u(t) in place before the force; the
+   changed u(t) is the next step's u(t-dt), as after the reference's copy. */
 __global__ void inject_kernel(const InstDesc *insts, const int *levels)
 {
     const InstDesc &D = insts[blockIdx.y];
@@ -363,12 +365,12 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
     if (it >= D.nst - 1) return;
     const int lev = it + 1;
     if (lev >= D.nt_sem) return;
-    const int p = it & 1, nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld, mt = 2 * (nx + nz), sht = nx + nz;
+    const int p = it % 3, nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld, mt = 2 * (nx + nz), sht = nx + nz;
     const float *g = D.G + (long)lev * mt, *g0 = D.G + (long)(lev - 1) * mt;
     const float *gv = D.Gv + (long)lev * sht, *gv0 = D.Gv + (long)(lev - 1) * sht;
     const bool sh = D.solver == SEMSKS_SOLVER_PSVSH, mp = D.solver == SEMSKS_SOLVER_PSV_MP;
     float *U = D.u[p];
-    float *S = D.st;
+    const float *Up = D.u[(p + 2) % 3];
     const int nb = nx - nbl, nl = nz - nbl - 1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb + nl; i += gridDim.x * blockDim.x) {
         long q;
@@ -377,9 +379,9 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
             int ix = nbl + i;
             q = (long)nbl * ld + ix;
             if (mp) {
-                d[0] = -(U[3 * q] - S[4 * q]) + g[ix];
+                d[0] = -Up[4 * q] + g[ix];
                 d[1] = 0.f;
-                d[2] = -(U[3 * q + 2] - S[4 * q + 2]) - g[ix + nx];
+                d[2] = -Up[4 * q + 2] - g[ix + nx];
             } else {
                 d[0] = -g0[ix] + g[ix];
                 d[1] = sh ? -gv0[ix] + gv[ix] : 0.f;
@@ -389,19 +391,19 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
             int iz = nbl + 1 + (i - nb);
             q = (long)iz * ld + nbl;
             if (mp) {
-                d[0] = -(U[3 * q] - S[4 * q]) + g[2 * nx + iz];
+                d[0] = -Up[4 * q] + g[2 * nx + iz];
                 d[1] = 0.f;
-                d[2] = -(U[3 * q + 2] - S[4 * q + 2]) - g[2 * nx + nz + iz];
+                d[2] = -Up[4 * q + 2] - g[2 * nx + nz + iz];
             } else {
                 d[0] = -g0[2 * nx + iz] + g[2 * nx + iz];
                 d[1] = sh ? -gv0[nx + iz] + gv[nx + iz] : 0.f;
                 d[2] = g0[2 * nx + nz + iz] - g[2 * nx + nz + iz];
             }
         }
-        const uint32_t w = D.kahan ? __float_as_uint(S[4 * q + 3]) : 0u;
+        const uint32_t w = D.kahan ? __float_as_uint(U[4 * q + 3]) : 0u;
         uint32_t wn = 0;
         for (int k = 0; k < 3; ++k) {
-            float u = U[3 * q + k];
+            float u = U[4 * q + k];
             if (D.kahan) {
                 float c = c_unpack(w, k, u);
                 kahan_add(u, c, d[k]);
@@ -409,25 +411,23 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
             } else {
                 u = u + d[k];
             }
-            U[3 * q + k] = u;
-            S[4 * q + k] = S[4 * q + k] + d[k];
+            U[4 * q + k] = u;
         }
-        if (D.kahan) S[4 * q + 3] = __uint_as_float(wn);
+        U[4 * q + 3] = __uint_as_float(wn);
     }
 }
 
 /* atten_wave2 (sem2d_psvsh.c:1780-1823): after the update, u(t+dt) on the
    edge strips of nh = neh*hpn nodes is multiplied by the taper (bottom for
    atten >= 1, left >= 2, right >= 3, top >= 4; each node at most once).
-   The reference then rotates the tapered u(t+dt) into u, so delta becomes
-   u'(t+dt) - u(t) and the compensation of the rescaled value restarts. */
+   The compensation of the rescaled value restarts. */
 __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
 {
     const InstDesc &D = insts[blockIdx.y];
     if (!D.active || D.atten <= 0) return;
     const int it = levels[blockIdx.y];
     if (it >= D.nst - 1) return;
-    const int p = it & 1, nh = D.nh, nx = D.nx, nz = D.nz, ld = D.ld, at = D.atten;
+    const int pn = (it + 1) % 3, nh = D.nh, nx = D.nx, nz = D.nz, ld = D.ld, at = D.atten;
     const long nb = (long)nh * nx, ns = (long)(nz - nh) * nh, ntop = (long)nh * (nx - 2 * nh);
     const long total = nb + (at > 1 ? ns : 0) + (at > 2 ? ns : 0) + (at > 3 ? ntop : 0);
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
@@ -449,15 +449,9 @@ __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
             f = D.taper[iz - nz + nh];
         }
         const long q = (long)iz * ld + ix;
-        float *U1 = D.u[p ^ 1] + 3 * q;
-        const float *U0 = D.u[p] + 3 * q;
-        float *S = D.st + 4 * q;
-        for (int k = 0; k < 3; ++k) {
-            const float un = f * U1[k];
-            S[k] = un - U0[k];
-            U1[k] = un;
-        }
-        S[3] = 0.f;
+        float *U1 = D.u[pn] + 4 * q;
+        for (int k = 0; k < 3; ++k) U1[k] = f * U1[k];
+        U1[3] = 0.f;
     }
 }
 
@@ -468,13 +462,13 @@ __global__ void record_kernel(const InstDesc *insts, int *levels)
     if (!D.active) return;
     const int it = levels[blockIdx.x];
     if (it >= D.nst - 1) return;
-    const int lev = it + 1, p = lev & 1, nt = D.ntrace;
+    const int lev = it + 1, p = lev % 3, nt = D.ntrace;
     float *tr = D.traces + (long)lev * 3 * nt;
     for (int r = threadIdx.x; r < nt; r += blockDim.x) {
         long q = D.rec[r];
-        tr[r] = D.u[p][3 * q];
-        tr[nt + r] = -D.u[p][3 * q + 2];
-        tr[2 * nt + r] = D.u[p][3 * q + 1];
+        tr[r] = D.u[p][4 * q];
+        tr[nt + r] = -D.u[p][4 * q + 2];
+        tr[2 * nt + r] = D.u[p][4 * q + 1];
     }
     __syncthreads();
     if (threadIdx.x == 0) levels[blockIdx.x] = lev;
@@ -485,9 +479,9 @@ __global__ void record0_kernel(InstDesc D)
     const int nt = D.ntrace;
     for (int r = threadIdx.x; r < nt; r += blockDim.x) {
         long q = D.rec[r];
-        D.traces[r] = D.u[0][3 * q];
-        D.traces[nt + r] = -D.u[0][3 * q + 2];
-        D.traces[2 * nt + r] = D.u[0][3 * q + 1];
+        D.traces[r] = D.u[0][4 * q];
+        D.traces[nt + r] = -D.u[0][4 * q + 2];
+        D.traces[2 * nt + r] = D.u[0][4 * q + 1];
     }
 }
 
@@ -588,17 +582,28 @@ __device__ __forceinline__ float mdt2_of(const InstDesc &D, int mid, int ix, int
     return gp(D.mclass)[(long)mid * NC * NC + mass_class<N>(iz, D.nz - 1) * NC + mass_class<N>(ix, D.nx - 1)];
 }
 
-/* one node: delta, Kahan-compensated u, edge rows by their Lysmer factors */
+/* delta of the previous step from the compensated values of u(t) and
+   u(t-dt): (u - up) - (c - cp), c the compensation (true value u - c) */
+__device__ __forceinline__ float delta_of(float u, uint32_t w, float up, uint32_t wp, int k, bool kahan)
+{
+    const float d = u - up;
+    return kahan ? d - (c_unpack(w, k, u) - c_unpack(wp, k, up)) : d;
+}
+
+/* one node: delta, Kahan-compensated u, edge rows by their Lysmer factors;
+   uc/w: the node's u(t) and its packed compensation; p: buffer of level t */
 template <int N>
 __device__ __forceinline__ void update_one(const InstDesc &D, int p, long q, int ix, int iz, int mid,
-                                           const float (&f)[3], const float (&uc)[3])
+                                           const float (&f)[3], const float (&uc)[3], uint32_t w)
 {
     const bool kahan = D.kahan != 0;
     const int e = edge_index(ix, iz, D.nx, D.nz, D.hpn);
     float d[3], ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
-    const float4 s4 = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.st) + 4 * q);
-    d[0] = s4.x; d[1] = s4.y; d[2] = s4.z;
-    const uint32_t w = kahan ? __float_as_uint(s4.w) : 0u;
+    const float4 pv = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.u[(p + 2) % 3]) + 4 * q);
+    const uint32_t wp = __float_as_uint(pv.w);
+    d[0] = delta_of(uc[0], w, pv.x, wp, 0, kahan);
+    d[1] = delta_of(uc[1], w, pv.y, wp, 1, kahan);
+    d[2] = delta_of(uc[2], w, pv.z, wp, 2, kahan);
     float m = 0.f;
     if (e >= 0) {
 #pragma unroll
@@ -620,9 +625,7 @@ __device__ __forceinline__ void update_one(const InstDesc &D, int p, long q, int
             nu[k] = nu[k] + d[k];
         }
     }
-    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.st) + 4 * q) = make_float4(d[0], d[1], d[2], __uint_as_float(wn));
-#pragma unroll
-    for (int k = 0; k < 3; ++k) gp(D.u[p ^ 1])[3 * q + k] = nu[k];
+    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[(p + 1) % 3]) + 4 * q) = make_float4(nu[0], nu[1], nu[2], __uint_as_float(wn));
 }
 
 /* Lane layout of the element-force/update kernel: a 64-lane wave holds
@@ -770,14 +773,16 @@ __device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)
 }
 
 /* What a lane fetches from global memory for one element row: its row of
-   u (columns 0..N-1, plus column N on lanes whose right neighbour lane does
-   not hold it), and, for the update, the row's delta, packed compensation
-   and dt^2/M.  The next row's fetch is issued before the current row's
-   stores, so waiting for it never waits for those stores. */
+   u(t) with the packed compensations (columns 0..N-1, plus column N on
+   lanes whose right neighbour lane does not hold it), and, for the update,
+   the row of u(t-dt) and dt^2/M.  The next row's fetch is issued before the
+   current row's stores, so waiting for it never waits for those stores. */
 template <int N> struct RowFetch {
     float u[N][3];
+    uint32_t c[N];     /* packed compensation of u(t) */
     float un[3];
-    float4 sd[4];      /* per owned node: delta xyz, packed compensation */
+    uint32_t cn;
+    float4 pv[4];      /* per owned node: u(t-dt) xyz, its packed compensation */
     float m[4];
     int rid;           /* single-record tiles: the row's material record */
 };
@@ -807,7 +812,8 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     const int ex_first = x0 > 0 ? x0 - 1 : 0;
     const RowCoef<N> rc = row_coef<N>(b);
 
-    const SEMSKS_GMEM float *U = gp(FORCE_ONLY ? D.u[0] : D.u[p]);
+    const SEMSKS_GMEM float *U = gp(D.u[p]);
+    const int pp = (p + 2) % 3, pn = (p + 1) % 3;       /* buffers of levels t-dt, t+dt */
 
     const int ez0 = z0 > 0 ? z0 - 1 : 0;
     const int ez1 = z0 + rows;
@@ -815,41 +821,32 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     auto fetch = [&](int ez, RowFetch<N> &r) {
         const int iz = ez * N + b;
         const long rowoff = (long)iz * ld + (long)exl * N;
-        /* N nodes x 3 components, contiguous; 16-byte aligned for N = 4 */
-        if (valid) {
-            if constexpr (N == 4) {
-                const SEMSKS_GMEM float4 *v4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(U + 3 * rowoff);
-                const float4 t0 = v4[0], t1 = v4[1], t2 = v4[2];
-                r.u[0][0] = t0.x; r.u[0][1] = t0.y; r.u[0][2] = t0.z; r.u[1][0] = t0.w;
-                r.u[1][1] = t1.x; r.u[1][2] = t1.y; r.u[2][0] = t1.z; r.u[2][1] = t1.w;
-                r.u[2][2] = t2.x; r.u[3][0] = t2.y; r.u[3][1] = t2.z; r.u[3][2] = t2.w;
-            } else {
+        /* N nodes of 16 bytes: x, y, z, packed compensation */
+        const SEMSKS_GMEM float4 *v4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(U + 4 * rowoff);
 #pragma unroll
-                for (int a = 0; a < N; ++a)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) r.u[a][k] = U[3 * (rowoff + a) + k];
-            }
-        } else {
-#pragma unroll
-            for (int a = 0; a < N; ++a)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) r.u[a][k] = 0.f;
+        for (int a = 0; a < N; ++a) {
+            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (valid) t = v4[a];
+            r.u[a][0] = t.x; r.u[a][1] = t.y; r.u[a][2] = t.z; r.c[a] = __float_as_uint(t.w);
         }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) r.un[k] = direct ? U[3 * (rowoff + N) + k] : 0.f;
+        {
+            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (direct) t = v4[N];
+            r.un[0] = t.x; r.un[1] = t.y; r.un[2] = t.z; r.cn = __float_as_uint(t.w);
+        }
         r.rid = 0;
         if constexpr (!MIXED) r.rid = __builtin_amdgcn_readfirstlane((int)gp(D.emat)[(long)ez * nex + ex_first]);
         if constexpr (N == 4 && !FORCE_ONLY) {
             const bool own = own_x && ez >= z0 && b < N;
             const bool vec = own && iz != 0 && ex != 0;
 #pragma unroll
-            for (int a = 0; a < 4; ++a) r.sd[a] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int a = 0; a < 4; ++a) r.pv[a] = make_float4(0.f, 0.f, 0.f, 0.f);
             r.m[0] = r.m[1] = r.m[2] = r.m[3] = 0.f;
             if (vec) {
                 const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
-                const SEMSKS_GMEM float4 *s4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.st) + 4 * rowoff);
+                const SEMSKS_GMEM float4 *s4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.u[pp]) + 4 * rowoff);
 #pragma unroll
-                for (int a = 0; a < 4; ++a) r.sd[a] = s4[a];
+                for (int a = 0; a < 4; ++a) r.pv[a] = s4[a];
                 /* mass class: cz = b (0 = row shared with the element below) */
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
@@ -959,38 +956,31 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
 #pragma unroll
                         for (int k = 0; k < 3; ++k) gp(D.fout[k])[q0 + a] = f[a][k];
             } else if (N == 4 && vec) {
-                float v[4][3];
-                float4 sn[4];
+                SEMSKS_GMEM float4 *u4 = reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[pn]) + 4 * q0);
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
-                    float d[3] = {cur.sd[a].x, cur.sd[a].y, cur.sd[a].z};
-                    const uint32_t wc = __float_as_uint(cur.sd[a].w);
+                    const uint32_t wc = cur.c[a], wp = __float_as_uint(cur.pv[a].w);
+                    const float up[3] = {cur.pv[a].x, cur.pv[a].y, cur.pv[a].z};
+                    float v[3];
                     uint32_t cw = 0u;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        d[k] = d[k] - cur.m[a] * f[a][k];
-                        v[a][k] = u[a][k];
+                        const float d = delta_of(u[a][k], wc, up[k], wp, k, kahan) - cur.m[a] * f[a][k];
+                        v[k] = u[a][k];
                         if (kahan) {
-                            float c = c_unpack(wc, k, v[a][k]);
-                            kahan_add(v[a][k], c, d[k]);
-                            cw |= c_pack(c, v[a][k]) << (8 * k);
+                            float c = c_unpack(wc, k, v[k]);
+                            kahan_add(v[k], c, d);
+                            cw |= c_pack(c, v[k]) << (8 * k);
                         } else {
-                            v[a][k] = v[a][k] + d[k];
+                            v[k] = v[k] + d;
                         }
                     }
-                    sn[a] = make_float4(d[0], d[1], d[2], __uint_as_float(cw));
+                    u4[a] = make_float4(v[0], v[1], v[2], __uint_as_float(cw));
                 }
-                SEMSKS_GMEM float4 *s4 = reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.st) + 4 * q0);
-#pragma unroll
-                for (int a = 0; a < 4; ++a) s4[a] = sn[a];
-                SEMSKS_GMEM float4 *u4 = reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[p ^ 1]) + 3 * q0);
-                u4[0] = make_float4(v[0][0], v[0][1], v[0][2], v[1][0]);
-                u4[1] = make_float4(v[1][1], v[1][2], v[2][0], v[2][1]);
-                u4[2] = make_float4(v[2][2], v[3][0], v[3][1], v[3][2]);
                 if (last_x) {
                     float fa[3] = {f[N][0], f[N][1], f[N][2]};
                     float ua[3] = {u[N][0], u[N][1], u[N][2]};
-                    update_one<N>(D, p, q0 + N, ixb + N, iz, MIXED ? (int)idrow[N] : rid, fa, ua);
+                    update_one<N>(D, p, q0 + N, ixb + N, iz, MIXED ? (int)idrow[N] : rid, fa, ua, cur.cn);
                 }
             } else {
 #pragma unroll
@@ -998,7 +988,8 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
                     if (a >= a1) continue;
                     float fa[3] = {f[a][0], f[a][1], f[a][2]};
                     float ua[3] = {u[a][0], u[a][1], u[a][2]};
-                    update_one<N>(D, p, q0 + a, ixb + a, iz, MIXED ? (int)idrow[a] : rid, fa, ua);
+                    update_one<N>(D, p, q0 + a, ixb + a, iz, MIXED ? (int)idrow[a] : rid, fa, ua,
+                                  a < N ? cur.c[a] : cur.cn);
                 }
             }
         }
@@ -1030,7 +1021,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : 2) void step_kernel(const InstDes
         if constexpr (!FORCE_ONLY) {
             const int it = levels[tile.x];
             if (it >= D.nst - 1) continue;
-            p = it & 1;
+            p = it % 3;
         }
         float *lds = lds_blocks[threadIdx.x >> 6];
         if (tile.w >> 16) process_tile<N, FORCE_ONLY, true>(D, p, tile.y, tile.z, tile.w & 0xFFFF, lane, lds);
@@ -1400,7 +1391,7 @@ static int nt_sem_of(const semsks_problem *p) { return (int)((float)((p->nt - 3)
 size_t semsks_footprint(const semsks_problem *p, int shared)
 {
     const size_t nodes = (size_t)ld_of(p) * nz_of(p);
-    size_t b = nodes * 4 * (6 + 4);                 /* u(t), u(t+dt), delta + packed compensation */
+    size_t b = nodes * 4 * (3 * 4);                 /* three levels of {x, y, z, packed compensation} */
     if (!shared) b += nodes * 2 + (size_t)p->nex * p->nez * 2 +
                       (size_t)p->n_mat * (64 + 4 * (p->hpn + 1) * (p->hpn + 1)) +
                       (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24;
@@ -1489,12 +1480,10 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
 #define ACHK(x) do { if ((x) != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "device allocation failed"); goto fail; } } while (0)
     {
         float *st_all = nullptr;
-        const int nplanes = 6 + 4;
+        const int nplanes = 3 * 4;                   /* three levels of 16-byte records */
         ACHK(dalloc(s, &st_all, nodes * nplanes));
         ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->stream));
-        d.u[0] = st_all;
-        d.u[1] = st_all + 3 * nodes;
-        d.st = st_all + 6 * nodes;
+        for (int l = 0; l < 3; ++l) d.u[l] = st_all + 4 * l * nodes;
     }
     if (p->atten > 0) {   /* atten_taper (sem2d_psvsh.c:1725-1738), double then float */
         const int nh = p->neh * p->hpn;
@@ -1809,14 +1798,14 @@ semsks_status semsks_get_field(semsks_engine *e, int k, float *out)
 {
     if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
     const Slot &s = e->slots[k];
-    const int p = e->h_levels[k] & 1;
-    std::vector<float> aos((size_t)s.ld * s.nz * 3);
+    const int p = e->h_levels[k] % 3;
+    std::vector<float> aos((size_t)s.ld * s.nz * 4);
     HIPCHK(hipMemcpyAsync(aos.data(), s.d.u[p], aos.size() * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     for (int c = 0; c < 3; ++c)
         for (int iz = 0; iz < s.nz; ++iz)
             for (int ix = 0; ix < s.nx; ++ix)
-                out[((size_t)c * s.nz + iz) * s.nx + ix] = aos[((size_t)iz * s.ld + ix) * 3 + c];
+                out[((size_t)c * s.nz + iz) * s.nx + ix] = aos[((size_t)iz * s.ld + ix) * 4 + c];
     return SEMSKS_OK;
 }
 
@@ -1826,16 +1815,16 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     const Slot &s = e->slots[k];
     const size_t nodes = (size_t)s.ld * s.nz;
     float *buf = nullptr;
-    HIPCHK(hipMalloc(&buf, nodes * 6 * sizeof(float) + 256));
-    HIPCHK(hipMemsetAsync(buf, 0, nodes * 6 * sizeof(float), e->stream));
+    HIPCHK(hipMalloc(&buf, nodes * 7 * sizeof(float) + 256));
+    HIPCHK(hipMemsetAsync(buf, 0, nodes * 7 * sizeof(float), e->stream));
     InstDesc d = s.d;
-    std::vector<float> aos(nodes * 3, 0.f);
+    std::vector<float> aos(nodes * 4, 0.f);
     for (int c = 0; c < 3; ++c)
         for (int iz = 0; iz < s.nz; ++iz)
             for (int ix = 0; ix < s.nx; ++ix)
-                aos[((size_t)iz * s.ld + ix) * 3 + c] = u[((size_t)c * s.nz + iz) * s.nx + ix];
+                aos[((size_t)iz * s.ld + ix) * 4 + c] = u[((size_t)c * s.nz + iz) * s.nx + ix];
     d.u[0] = buf;
-    for (int c = 0; c < 3; ++c) d.fout[c] = buf + (3 + c) * nodes;
+    for (int c = 0; c < 3; ++c) d.fout[c] = buf + (4 + c) * nodes;
     HIPCHK(hipMemcpyAsync(d.u[0], aos.data(), aos.size() * 4, hipMemcpyHostToDevice, e->stream));
     d.active = 1;
     InstDesc *dd = nullptr;

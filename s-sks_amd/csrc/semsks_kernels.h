@@ -29,11 +29,10 @@
 struct InstDesc {
     int nex, nez, nx, nz, ld;
     int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active;
-    /* displacement u(t) and u(t+dt), xyz interleaved: node q at u[p][3q..3q+2] */
-    float *u[2];
-    /* update state, one 16-byte record per node: delta = u(t) - u(t-dt)
-       (xyz) and the packed Kahan compensation of u (bits) */
-    float *st;
+    /* displacement of three time levels, level it in u[it % 3]; one
+       16-byte record per node: x, y, z and the packed Kahan compensation
+       of the three components (bits): node q at u[l][4q..4q+3] */
+    float *u[3];
     float *fout[3];                 /* force-only mode output planes, else null */
     const uint16_t *mat_id;
     const uint16_t *emat;
