@@ -66,13 +66,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+STEPS_PER_SWEEP = 1     # engine option: 1 = one time step per launch, 2 = two-step sweeps
+
+
 def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0):
     """Admits one instance per (built, p, psi) entry (boundary records made by
     the device plane-wave generator), runs warmup + timed steps in hipGraph
     mode.  Returns throughput (pt-steps/s), ms/step and the live engine."""
     import semsks
     nt = warmup + steps + extra_steps + 8
-    eng = semsks.Engine(max_slots=len(waves), tile_z=tile_z, use_graph=1)
+    eng = semsks.Engine(max_slots=len(waves), tile_z=tile_z, use_graph=1, steps_per_sweep=STEPS_PER_SWEEP)
     slots = []
     t0 = time.time()
     for k, (b, p, psi) in enumerate(waves):
@@ -171,9 +174,13 @@ def main():
     ap.add_argument("--only", choices=["plain", "batched"], default=None,
                     help="c3: run only one of the two legs (for profiler runs)")
     ap.add_argument("--kernel-steps", type=int, default=10)
+    ap.add_argument("--steps-per-sweep", type=int, default=1, choices=[1, 2],
+                    help="2: two time steps per sweep of the mesh (step2_kernel) where possible")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one device copy of this many MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
+    global STEPS_PER_SWEEP
+    STEPS_PER_SWEEP = args.steps_per_sweep
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 4

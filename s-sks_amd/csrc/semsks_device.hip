@@ -32,10 +32,23 @@
  * node is written by exactly one lane, which applies the update in the same
  * pass while the row of u is still in registers.
  *
- * State in HBM: three time levels of u rotating (level it in buffer it % 3),
+ * Two steps per sweep (step2_kernel, N = 4, engine option): the wave keeps
+ * u(t+dt) of the rows it has just updated in registers and, one element row
+ * behind, takes the second step from them, so a sweep reads levels it and
+ * it-1 once and writes levels it+1 and it+2.  Its tiles own 9 of the 12
+ * element columns and sweep two halo rows below and one above; u(t+dt) on
+ * the interface row/column gets the next step's injection in registers.
+ * It moves 35% fewer bytes per point-step but issues ~1.5x the vector
+ * instructions (halo columns and rows, two live rows of state), and the
+ * one-step kernel already keeps the vector ALUs ~2/3 busy, so one step per
+ * launch is the default.
+ *
+ * State in HBM: four time levels of u rotating (level it in buffer it % 4),
  * one 16-byte record per node {x, y, z, packed Kahan compensation}.  A step
- * reads level it (with halos) and level it-1 (owned nodes) and writes level
- * it+1.  Rows are padded to a multiple of 64 nodes.
+ * reads level it (with halos) and level it-1 and writes level it+1 (and
+ * it+2 in a two-step sweep, which therefore never overwrites a level that a
+ * neighbouring tile may still read).  Rows are padded to a multiple of 64
+ * nodes.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -333,87 +346,111 @@ __device__ __forceinline__ void kahan_add(float &u, float &c, float x)
 
 /* The Kahan compensation c of u is the rounding error of the last addition,
    |c| <= ulp(u)/2.  It is stored per component as a signed byte in units of
-   ulp(u)/256, three components per 32-bit word; the residual it leaves is
-   below ulp(u)/512 per step. */
-__device__ __forceinline__ int c_exponent(float u)
-{
-    const int e = (int)((__float_as_uint(u) >> 23) & 0xffu);
-    return e > 0 ? e : 1;
-}
-
+   ulp(u)/256 (2^(e-32) with u = m 2^e, 0.5 <= |m| < 1), three components
+   per 32-bit word; the residual it leaves is below ulp(u)/512 per step. */
 __device__ __forceinline__ float c_unpack(uint32_t w, int k, float u)
 {
     const int q = (int)(int8_t)((w >> (8 * k)) & 0xffu);
-    return ldexpf((float)q, c_exponent(u) - 158);
+    return ldexpf((float)q, __builtin_amdgcn_frexp_expf(u) - 32);
 }
 
-__device__ __forceinline__ uint32_t c_pack(float c, float u)
+__device__ __forceinline__ int c_quant(float c, float u)
 {
-    int q = (int)rintf(ldexpf(c, 158 - c_exponent(u)));
-    q = q > 127 ? 127 : (q < -127 ? -127 : q);
-    return (uint32_t)q & 0xffu;
+    const int q = (int)rintf(ldexpf(c, 32 - __builtin_amdgcn_frexp_expf(u)));
+    return q > 127 ? 127 : (q < -127 ? -127 : q);
+}
+
+__device__ __forceinline__ uint32_t c_pack3(int q0, int q1, int q2)
+{
+    const uint32_t t = __builtin_amdgcn_perm((uint32_t)q1, (uint32_t)q0, 0x0c0c0400u);   /* q0.b0, q1.b0 */
+    return __builtin_amdgcn_perm((uint32_t)q2, t, 0x0c040100u);                         /* t.b0, t.b1, q2.b0 */
+}
+
+/* u += d for the three components, the compensation packed in w carried
+   along (w = 0 without it) */
+__device__ __forceinline__ void kahan3(bool kahan, float (&u)[3], uint32_t &w, const float (&d)[3])
+{
+    if (kahan) {
+        int q[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float c = c_unpack(w, k, u[k]);
+            kahan_add(u[k], c, d[k]);
+            q[k] = c_quant(c, u[k]);
+        }
+        w = c_pack3(q[0], q[1], q[2]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) u[k] = u[k] + d[k];
+        w = 0u;
+    }
 }
 
 /* GRT_Interface4: u += g(t) - g(t-dt) (w sign flipped); GRT_Interface3:
    u += -u(t-dt) + g(t).  Both change u(t) in place before the force; the
-   changed u(t) is the next step's u(t-dt), as after the reference's copy. */
+   changed u(t) is the next step's u(t-dt), as after the reference's copy.
+   inject_delta gives the increment at one interface node for the step that
+   leaves level it: the bottom row iz = nbl (ix >= nbl, the corner included)
+   or the left column ix = nbl (iz > nbl); up0/up2 are u_x/u_z of level it-1
+   at the node (GRT_Interface3 only). */
+__device__ __forceinline__ void inject_delta(const InstDesc &D, int it, int ix, int iz, float up0, float up2, float (&d)[3])
+{
+    const int nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, mt = 2 * (nx + nz), sht = nx + nz;
+    const int lev = it + 1;
+    const float *g = D.G + (long)lev * mt, *g0 = D.G + (long)(lev - 1) * mt;
+    const float *gv = D.Gv + (long)lev * sht, *gv0 = D.Gv + (long)(lev - 1) * sht;
+    const bool sh = D.solver == SEMSKS_SOLVER_PSVSH, mp = D.solver == SEMSKS_SOLVER_PSV_MP;
+    if (iz == nbl) {
+        if (mp) {
+            d[0] = -up0 + g[ix];
+            d[1] = 0.f;
+            d[2] = -up2 - g[ix + nx];
+        } else {
+            d[0] = -g0[ix] + g[ix];
+            d[1] = sh ? -gv0[ix] + gv[ix] : 0.f;
+            d[2] = g0[ix + nx] - g[ix + nx];
+        }
+    } else {
+        if (mp) {
+            d[0] = -up0 + g[2 * nx + iz];
+            d[1] = 0.f;
+            d[2] = -up2 - g[2 * nx + nz + iz];
+        } else {
+            d[0] = -g0[2 * nx + iz] + g[2 * nx + iz];
+            d[1] = sh ? -gv0[nx + iz] + gv[nx + iz] : 0.f;
+            d[2] = g0[2 * nx + nz + iz] - g[2 * nx + nz + iz];
+        }
+    }
+}
+
+__device__ __forceinline__ bool is_inject_node(int ix, int iz, int nbl)
+{
+    return (iz == nbl && ix >= nbl) || (ix == nbl && iz > nbl);
+}
+
 __global__ void inject_kernel(const InstDesc *insts, const int *levels)
 {
     const InstDesc &D = insts[blockIdx.y];
     if (!D.active) return;
     const int it = levels[blockIdx.y];
     if (it >= D.nst - 1) return;
-    const int lev = it + 1;
-    if (lev >= D.nt_sem) return;
-    const int p = it % 3, nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld, mt = 2 * (nx + nz), sht = nx + nz;
-    const float *g = D.G + (long)lev * mt, *g0 = D.G + (long)(lev - 1) * mt;
-    const float *gv = D.Gv + (long)lev * sht, *gv0 = D.Gv + (long)(lev - 1) * sht;
-    const bool sh = D.solver == SEMSKS_SOLVER_PSVSH, mp = D.solver == SEMSKS_SOLVER_PSV_MP;
+    if (it + 1 >= D.nt_sem) return;
+    const int p = it & 3, nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld;
+    const bool mp = D.solver == SEMSKS_SOLVER_PSV_MP;
     float *U = D.u[p];
-    const float *Up = D.u[(p + 2) % 3];
+    const float *Up = D.u[(p + 3) & 3];
     const int nb = nx - nbl, nl = nz - nbl - 1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb + nl; i += gridDim.x * blockDim.x) {
-        long q;
+        const int ix = i < nb ? nbl + i : nbl;
+        const int iz = i < nb ? nbl : nbl + 1 + (i - nb);
+        const long q = (long)iz * ld + ix;
         float d[3];
-        if (i < nb) {
-            int ix = nbl + i;
-            q = (long)nbl * ld + ix;
-            if (mp) {
-                d[0] = -Up[4 * q] + g[ix];
-                d[1] = 0.f;
-                d[2] = -Up[4 * q + 2] - g[ix + nx];
-            } else {
-                d[0] = -g0[ix] + g[ix];
-                d[1] = sh ? -gv0[ix] + gv[ix] : 0.f;
-                d[2] = g0[ix + nx] - g[ix + nx];
-            }
-        } else {
-            int iz = nbl + 1 + (i - nb);
-            q = (long)iz * ld + nbl;
-            if (mp) {
-                d[0] = -Up[4 * q] + g[2 * nx + iz];
-                d[1] = 0.f;
-                d[2] = -Up[4 * q + 2] - g[2 * nx + nz + iz];
-            } else {
-                d[0] = -g0[2 * nx + iz] + g[2 * nx + iz];
-                d[1] = sh ? -gv0[nx + iz] + gv[nx + iz] : 0.f;
-                d[2] = g0[2 * nx + nz + iz] - g[2 * nx + nz + iz];
-            }
-        }
-        const uint32_t w = D.kahan ? __float_as_uint(U[4 * q + 3]) : 0u;
-        uint32_t wn = 0;
-        for (int k = 0; k < 3; ++k) {
-            float u = U[4 * q + k];
-            if (D.kahan) {
-                float c = c_unpack(w, k, u);
-                kahan_add(u, c, d[k]);
-                wn |= c_pack(c, u) << (8 * k);
-            } else {
-                u = u + d[k];
-            }
-            U[4 * q + k] = u;
-        }
-        U[4 * q + 3] = __uint_as_float(wn);
+        inject_delta(D, it, ix, iz, mp ? Up[4 * q] : 0.f, mp ? Up[4 * q + 2] : 0.f, d);
+        float u[3] = {U[4 * q], U[4 * q + 1], U[4 * q + 2]};
+        uint32_t w = D.kahan ? __float_as_uint(U[4 * q + 3]) : 0u;
+        kahan3(D.kahan != 0, u, w, d);
+        for (int k = 0; k < 3; ++k) U[4 * q + k] = u[k];
+        U[4 * q + 3] = __uint_as_float(w);
     }
 }
 
@@ -427,7 +464,7 @@ __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
     if (!D.active || D.atten <= 0) return;
     const int it = levels[blockIdx.y];
     if (it >= D.nst - 1) return;
-    const int pn = (it + 1) % 3, nh = D.nh, nx = D.nx, nz = D.nz, ld = D.ld, at = D.atten;
+    const int pn = (it + 1) & 3, nh = D.nh, nx = D.nx, nz = D.nz, ld = D.ld, at = D.atten;
     const long nb = (long)nh * nx, ns = (long)(nz - nh) * nh, ntop = (long)nh * (nx - 2 * nh);
     const long total = nb + (at > 1 ? ns : 0) + (at > 2 ? ns : 0) + (at > 3 ? ntop : 0);
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
@@ -455,23 +492,27 @@ __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
     }
 }
 
-/* kirrecord_zhl: u to seis.u, -w to seis.w, v to seis.v; then advance the level */
-__global__ void record_kernel(const InstDesc *insts, int *levels)
+/* kirrecord_zhl: u to seis.u, -w to seis.w, v to seis.v for each level
+   the step launch produced (up to smax), then advance the level */
+__global__ void record_kernel(const InstDesc *insts, int *levels, int smax)
 {
     const InstDesc &D = insts[blockIdx.x];
     if (!D.active) return;
     const int it = levels[blockIdx.x];
     if (it >= D.nst - 1) return;
-    const int lev = it + 1, p = lev % 3, nt = D.ntrace;
-    float *tr = D.traces + (long)lev * 3 * nt;
-    for (int r = threadIdx.x; r < nt; r += blockDim.x) {
-        long q = D.rec[r];
-        tr[r] = D.u[p][4 * q];
-        tr[nt + r] = -D.u[p][4 * q + 2];
-        tr[2 * nt + r] = D.u[p][4 * q + 1];
+    const int adv = min(smax, D.nst - 1 - it), nt = D.ntrace;
+    for (int l = 1; l <= adv; ++l) {
+        const int lev = it + l, p = lev & 3;
+        float *tr = D.traces + (long)lev * 3 * nt;
+        for (int r = threadIdx.x; r < nt; r += blockDim.x) {
+            long q = D.rec[r];
+            tr[r] = D.u[p][4 * q];
+            tr[nt + r] = -D.u[p][4 * q + 2];
+            tr[2 * nt + r] = D.u[p][4 * q + 1];
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) levels[blockIdx.x] = lev;
+    if (threadIdx.x == 0) levels[blockIdx.x] = it + adv;
 }
 
 __global__ void record0_kernel(InstDesc D)
@@ -544,14 +585,34 @@ __device__ __forceinline__ constexpr float opW(int a) { constexpr ElemOps<N> o{}
 /* stress at one node from the 5x5 block C over {xx, zz, yz, xz, xy} and the
    2-D strains (uy,y = ux,y = uz,y = 0, Stress_ij sem2d_psvsh.c:1345-1379);
    C: 0:00 1:02 2:03 3:04 4:05 5:22 6:23 7:24 8:25 9:33 10:34 11:35 12:44 13:45 14:55 */
+template <bool ORTHO>
 __device__ __forceinline__ void stress5(const float *C, float e0, float e2, float e3, float e4, float e5,
                                         float &s0, float &s2, float &s3, float &s4, float &s5)
 {
-    s0 = C[0] * e0 + C[1] * e2 + C[2] * e3 + C[3] * e4 + C[4] * e5;
-    s2 = C[1] * e0 + C[5] * e2 + C[6] * e3 + C[7] * e4 + C[8] * e5;
-    s3 = C[2] * e0 + C[6] * e2 + C[9] * e3 + C[10] * e4 + C[11] * e5;
-    s4 = C[3] * e0 + C[7] * e2 + C[10] * e3 + C[12] * e4 + C[13] * e5;
-    s5 = C[4] * e0 + C[8] * e2 + C[11] * e3 + C[13] * e4 + C[14] * e5;
+    if constexpr (ORTHO) {
+        /* orthotropic in the x-y-z axes (isotropic, vertical or horizontal
+           axis along x or z): C 2,3,4,6,7,8,10,11,13 are zero */
+        s0 = C[0] * e0 + C[1] * e2;
+        s2 = C[1] * e0 + C[5] * e2;
+        s3 = C[9] * e3;
+        s4 = C[12] * e4;
+        s5 = C[14] * e5;
+    } else {
+        s0 = C[0] * e0 + C[1] * e2 + C[2] * e3 + C[3] * e4 + C[4] * e5;
+        s2 = C[1] * e0 + C[5] * e2 + C[6] * e3 + C[7] * e4 + C[8] * e5;
+        s3 = C[2] * e0 + C[6] * e2 + C[9] * e3 + C[10] * e4 + C[11] * e5;
+        s4 = C[3] * e0 + C[7] * e2 + C[10] * e3 + C[12] * e4 + C[13] * e5;
+        s5 = C[4] * e0 + C[8] * e2 + C[11] * e3 + C[13] * e4 + C[14] * e5;
+    }
+}
+
+/* record entries that are zero for an orthotropic medium in these axes */
+__host__ __device__ inline bool semsks_ortho_record(const float *C)
+{
+    const int z[9] = {2, 3, 4, 6, 7, 8, 10, 11, 13};
+    for (int i = 0; i < 9; ++i)
+        if (C[z[i]] != 0.f) return false;
+    return true;
 }
 
 __device__ __forceinline__ int edge_index(int ix, int iz, int nx, int nz, int n)
@@ -590,42 +651,44 @@ __device__ __forceinline__ float delta_of(float u, uint32_t w, float up, uint32_
     return kahan ? d - (c_unpack(w, k, u) - c_unpack(wp, k, up)) : d;
 }
 
+/* u(t+dt) of one node: delta from u(t) (u, packed compensation w) and
+   u(t-dt) (up, wp), then interior nodes (eidx < 0) by dt^2/M = m, edge
+   nodes by their Lysmer factors, and the Kahan-compensated sum */
+__device__ __forceinline__ void next_node(const InstDesc &D, bool kahan, int eidx, float m, const float (&f)[3],
+                                          const float (&uc)[3], uint32_t w, const float (&up)[3], uint32_t wp,
+                                          float (&nu)[3], uint32_t &wn)
+{
+    float d[3], ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
+    d[0] = delta_of(uc[0], w, up[0], wp, 0, kahan);
+    d[1] = delta_of(uc[1], w, up[1], wp, 1, kahan);
+    d[2] = delta_of(uc[2], w, up[2], wp, 2, kahan);
+    if (eidx >= 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { ea[k] = gp(D.edge_a)[eidx * 3 + k]; eb[k] = gp(D.edge_b)[eidx * 3 + k]; }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        d[k] = eidx < 0 ? d[k] - m * f[k] : ea[k] * d[k] - eb[k] * f[k];
+        nu[k] = uc[k];
+    }
+    wn = w;
+    kahan3(kahan, nu, wn, d);
+}
+
 /* one node: delta, Kahan-compensated u, edge rows by their Lysmer factors;
    uc/w: the node's u(t) and its packed compensation; p: buffer of level t */
 template <int N>
 __device__ __forceinline__ void update_one(const InstDesc &D, int p, long q, int ix, int iz, int mid,
                                            const float (&f)[3], const float (&uc)[3], uint32_t w)
 {
-    const bool kahan = D.kahan != 0;
     const int e = edge_index(ix, iz, D.nx, D.nz, D.hpn);
-    float d[3], ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
-    const float4 pv = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.u[(p + 2) % 3]) + 4 * q);
-    const uint32_t wp = __float_as_uint(pv.w);
-    d[0] = delta_of(uc[0], w, pv.x, wp, 0, kahan);
-    d[1] = delta_of(uc[1], w, pv.y, wp, 1, kahan);
-    d[2] = delta_of(uc[2], w, pv.z, wp, 2, kahan);
-    float m = 0.f;
-    if (e >= 0) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { ea[k] = gp(D.edge_a)[e * 3 + k]; eb[k] = gp(D.edge_b)[e * 3 + k]; }
-    } else {
-        m = mdt2_of<N>(D, mid, ix, iz);
-    }
+    const float4 pv = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.u[(p + 3) & 3]) + 4 * q);
+    const float up[3] = {pv.x, pv.y, pv.z};
+    const float m = e < 0 ? mdt2_of<N>(D, mid, ix, iz) : 0.f;
     float nu[3];
-    uint32_t wn = 0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        d[k] = e < 0 ? d[k] - m * f[k] : ea[k] * d[k] - eb[k] * f[k];
-        nu[k] = uc[k];
-        if (kahan) {
-            float c = c_unpack(w, k, nu[k]);
-            kahan_add(nu[k], c, d[k]);
-            wn |= c_pack(c, nu[k]) << (8 * k);
-        } else {
-            nu[k] = nu[k] + d[k];
-        }
-    }
-    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[(p + 1) % 3]) + 4 * q) = make_float4(nu[0], nu[1], nu[2], __uint_as_float(wn));
+    uint32_t wn;
+    next_node(D, D.kahan != 0, e, m, f, uc, w, up, __float_as_uint(pv.w), nu, wn);
+    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[(p + 1) & 3]) + 4 * q) = make_float4(nu[0], nu[1], nu[2], __uint_as_float(wn));
 }
 
 /* Lane layout of the element-force/update kernel: a 64-lane wave holds
@@ -678,7 +741,7 @@ __device__ __forceinline__ RowCoef<N> row_coef(int b)
     return c;
 }
 
-template <bool UNIFORM>
+template <bool UNIFORM, bool ORTHO>
 __device__ __forceinline__ void node_stress(const float *Cu, const SEMSKS_GMEM float *crec, float e0, float e2, float e3,
                                             float e4, float e5, float &s0, float &s2, float &s3, float &s4, float &s5)
 {
@@ -693,7 +756,7 @@ __device__ __forceinline__ void node_stress(const float *Cu, const SEMSKS_GMEM f
         C[7] = r2.x; C[8] = r2.y; C[9] = r2.z; C[10] = r2.w; C[11] = r3.x; C[12] = r3.y; C[13] = r3.z;
         C[14] = r3.w;
     }
-    stress5(C, e0, e2, e3, e4, e5, s0, s2, s3, s4, s5);
+    stress5<ORTHO>(C, e0, e2, e3, e4, e5, s0, s2, s3, s4, s5);
 }
 
 /* one element row of K_e u_e (Stiff_Matrix: dUj_di_2D, Stress_ij, F_ik,
@@ -701,7 +764,7 @@ __device__ __forceinline__ void node_stress(const float *Cu, const SEMSKS_GMEM f
    element's LDS blocks are laid out [a][k][t] (CB floats per column a), so a
    lane reads column a of all rows with a few 16-byte loads: Ub holds the
    rows of u, Qb the column tractions. */
-template <int N, bool UNIFORM>
+template <int N, bool UNIFORM, bool ORTHO>
 __device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)[N + 1][3], int b,
                                           const RowCoef<N> &rc, float *Ub, float *Qb, const float *Cu,
                                           const SEMSKS_GMEM float *mat, const SEMSKS_GMEM uint16_t *idrow, bool wr)
@@ -736,7 +799,7 @@ __device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)
         }
         float s0, s2, s3, s4, s5;
         const SEMSKS_GMEM float *crec = UNIFORM ? nullptr : mat + (long)idrow[a] * SEMSKS_MAT_STRIDE;
-        node_stress<UNIFORM>(Cu, crec, gx0, gz2, gz1, gx2 + gz0, gx1, s0, s2, s3, s4, s5);
+        node_stress<UNIFORM, ORTHO>(Cu, crec, gx0, gz2, gz1, gx2 + gz0, gx1, s0, s2, s3, s4, s5);
         const float wa = opW<N>(a);
         const float p0 = rc.wb * s0, p1 = rc.wb * s5, p2 = rc.wb * s4;   /* row traction    T_i0 */
 #pragma unroll
@@ -790,7 +853,7 @@ template <int N> struct RowFetch {
 /* MIXED = false: every element row of the tile (over the wave's element
    columns) holds one material record, which the host has checked; the
    record sits in scalar registers.  MIXED = true: material per node. */
-template <int N, bool FORCE_ONLY, bool MIXED>
+template <int N, bool FORCE_ONLY, bool MIXED, bool ORTHO>
 __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows, int lane, float *lds)
 {
     using RL = RowLayout<N>;
@@ -813,7 +876,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     const RowCoef<N> rc = row_coef<N>(b);
 
     const SEMSKS_GMEM float *U = gp(D.u[p]);
-    const int pp = (p + 2) % 3, pn = (p + 1) % 3;       /* buffers of levels t-dt, t+dt */
+    const int pp = (p + 3) & 3, pn = (p + 1) & 3;       /* buffers of levels t-dt, t+dt */
 
     const int ez0 = z0 > 0 ? z0 - 1 : 0;
     const int ez1 = z0 + rows;
@@ -898,9 +961,9 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
             float C[15];
 #pragma unroll
             for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
-            row_force<N, true>(u, f, b, rc, Ub, Qb, C, gp(D.mat), idrow, act);
+            row_force<N, true, ORTHO>(u, f, b, rc, Ub, Qb, C, gp(D.mat), idrow, act);
         } else {
-            row_force<N, false>(u, f, b, rc, Ub, Qb, nullptr, gp(D.mat), idrow, act);
+            row_force<N, false, ORTHO>(u, f, b, rc, Ub, Qb, nullptr, gp(D.mat), idrow, act);
         }
         if (!valid) {
 #pragma unroll
@@ -961,20 +1024,14 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
                 for (int a = 0; a < 4; ++a) {
                     const uint32_t wc = cur.c[a], wp = __float_as_uint(cur.pv[a].w);
                     const float up[3] = {cur.pv[a].x, cur.pv[a].y, cur.pv[a].z};
-                    float v[3];
-                    uint32_t cw = 0u;
+                    float v[3], d[3];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const float d = delta_of(u[a][k], wc, up[k], wp, k, kahan) - cur.m[a] * f[a][k];
+                        d[k] = delta_of(u[a][k], wc, up[k], wp, k, kahan) - cur.m[a] * f[a][k];
                         v[k] = u[a][k];
-                        if (kahan) {
-                            float c = c_unpack(wc, k, v[k]);
-                            kahan_add(v[k], c, d);
-                            cw |= c_pack(c, v[k]) << (8 * k);
-                        } else {
-                            v[k] = v[k] + d;
-                        }
                     }
+                    uint32_t cw = wc;
+                    kahan3(kahan, v, cw, d);
                     u4[a] = make_float4(v[0], v[1], v[2], __uint_as_float(cw));
                 }
                 if (last_x) {
@@ -1021,11 +1078,354 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : 2) void step_kernel(const InstDes
         if constexpr (!FORCE_ONLY) {
             const int it = levels[tile.x];
             if (it >= D.nst - 1) continue;
-            p = it % 3;
+            p = it & 3;
         }
         float *lds = lds_blocks[threadIdx.x >> 6];
-        if (tile.w >> 16) process_tile<N, FORCE_ONLY, true>(D, p, tile.y, tile.z, tile.w & 0xFFFF, lane, lds);
-        else process_tile<N, FORCE_ONLY, false>(D, p, tile.y, tile.z, tile.w, lane, lds);
+        const int rows = tile.w & 0xFFFF;
+        if (D.ortho) {
+            if (tile.w >> 16) process_tile<N, FORCE_ONLY, true, true>(D, p, tile.y, tile.z, rows, lane, lds);
+            else process_tile<N, FORCE_ONLY, false, true>(D, p, tile.y, tile.z, rows, lane, lds);
+        } else {
+            if (tile.w >> 16) process_tile<N, FORCE_ONLY, true, false>(D, p, tile.y, tile.z, rows, lane, lds);
+            else process_tile<N, FORCE_ONLY, false, false>(D, p, tile.y, tile.z, rows, lane, lds);
+        }
+    }
+}
+
+/* Two time steps per sweep (hpn = 4).  The tile owns element columns
+   x0..x0+8 and element rows z0..z0+rows-1; the wave holds element columns
+   x0-2..x0+9 (lane layout as process_tile).  First step (level it -> it+1)
+   on element rows z0-2..z0+rows: u(t+dt) is then valid on every node of
+   element columns x0-1..x0+8 and of element rows z0-1..z0+rows-1 (plus the
+   bottom node row of row z0+rows), which is all that the second step
+   (it+1 -> it+2), taken one element row behind on rows z0-1..z0+rows-1,
+   reads.  Each lane keeps its row of u(t+dt) and of u(t) in registers for
+   the second step; the top node row of an element comes from the lane that
+   holds it as the bottom row of the element above.  Before the second
+   step, u(t+dt) on the interface row/column gets the injection of level
+   it+1, exactly as inject_kernel would apply it in place.  Arithmetic is
+   the same, node by node, as two one-step launches.  two = false: only the
+   first step (an instance with one step left). */
+constexpr int XC2 = 9;
+
+template <bool MIXED, bool ORTHO>
+__device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0, int z0, int rows, int lane, float *lds)
+{
+    constexpr int N = 4;
+    using RL = RowLayout<N>;
+    constexpr int NP = RL::NP;
+    const int nex = D.nex, nez = D.nez, ld = D.ld, nx = D.nx, nz = D.nz;
+    const int e = lane / NP, b = lane - e * NP;
+    const bool act = lane < RL::NL;
+    const int ex = x0 - 2 + e;
+    const bool valid = act && ex >= 0 && ex < nex;
+    const bool own_x = valid && e >= 2 && e < 2 + XC2;
+    const bool last_x = ex == nex - 1;
+    const bool direct = valid && (e == RL::NE - 1 || last_x);
+    const bool kahan = D.kahan != 0;
+    const int ec = act ? e : 0;
+    float *Ub = lds + ec * RL::ESTR;
+    float *Qb = lds + RL::NE * RL::ESTR + ec * RL::ESTR;
+    float *CF1 = lds + 2 * RL::NE * RL::ESTR + ec * RL::CFS;
+    float *CF2 = lds + 2 * RL::NE * RL::ESTR + (RL::NE + ec) * RL::CFS;
+    const int exl = valid ? ex : 0;
+    const int ex_first = x0 >= 2 ? x0 - 2 : 0;
+    const RowCoef<N> rc = row_coef<N>(b);
+    const int nbl = D.neh * N;
+    const bool inj = two && it + 2 < D.nt_sem;   /* inject_kernel's test for level it+1 */
+
+    const SEMSKS_GMEM float *U = gp(D.u[p]);
+    const SEMSKS_GMEM float *UP = gp(D.u[(p + 3) & 3]);
+    SEMSKS_GMEM float *U1 = gp(D.u[(p + 1) & 3]);
+    SEMSKS_GMEM float *U2 = gp(D.u[(p + 2) & 3]);
+
+    const int r0 = z0 >= 2 ? z0 - 2 : 0;            /* first row of step 1 */
+    const int rs2 = z0 >= 1 ? z0 - 1 : 0;           /* first row of step 2 */
+    const int rend = two ? z0 + rows + 1 : z0 + rows;
+    const int r1 = rend < nez ? rend : nez;         /* step 1 on rows [r0, r1) */
+
+    auto fetch = [&](int ez, RowFetch<N> &r) {
+        const int iz = ez * N + b;
+        const long rowoff = (long)iz * ld + (long)exl * N;
+        const SEMSKS_GMEM float4 *v4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(U + 4 * rowoff);
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (valid) t = v4[a];
+            r.u[a][0] = t.x; r.u[a][1] = t.y; r.u[a][2] = t.z; r.c[a] = __float_as_uint(t.w);
+        }
+        {
+            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (direct) t = v4[N];
+            r.un[0] = t.x; r.un[1] = t.y; r.un[2] = t.z; r.cn = __float_as_uint(t.w);
+        }
+        r.rid = 0;
+        if constexpr (!MIXED) r.rid = __builtin_amdgcn_readfirstlane((int)gp(D.emat)[(long)ez * nex + ex_first]);
+    };
+    /* u(t-dt) and dt^2/M of the row, loaded at the top of its first step */
+    auto fetch_prev = [&](int ez, RowFetch<N> &r) {
+        const int iz = ez * N + b;
+        const long rowoff = (long)iz * ld + (long)exl * N;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) r.pv[a] = make_float4(0.f, 0.f, 0.f, 0.f);
+        r.m[0] = r.m[1] = r.m[2] = r.m[3] = 0.f;
+        if (valid && (b < N || ez == nez - 1)) {
+            const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
+            const SEMSKS_GMEM float4 *s4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(UP + 4 * rowoff);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) r.pv[a] = s4[a];
+            const bool inner = ex > 0 && iz != 0 && iz != nz - 1;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int mid = MIXED ? (int)idrow[a] : r.rid;
+                r.m[a] = inner ? gp(D.mclass)[(long)mid * 25 + b * 5 + a] : mdt2_of<N>(D, mid, ex * N + a, iz);
+            }
+        }
+    };
+
+    /* u(t+dt) of the lane's row: nodes 0..N-1 (and N on the last element
+       column) from u(t) = uu/wu, u(t-dt) = pv/pw, dt^2/M = m and the force;
+       lanes with no edge node take the interior form directly */
+    auto row_next = [&](int iz, const float (&f)[NP][3], const float (&uu)[NP][3], const uint32_t (&wu)[NP],
+                        const float (&pv)[NP][3], const uint32_t (&pw)[NP], const float (&m)[4],
+                        float (&nu)[NP][3], uint32_t (&nw)[NP]) {
+        if (ex > 0 && iz != 0 && iz != nz - 1) {
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                float d[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    d[k] = delta_of(uu[a][k], wu[a], pv[a][k], pw[a], k, kahan) - m[a] * f[a][k];
+                    nu[a][k] = uu[a][k];
+                }
+                nw[a] = wu[a];
+                kahan3(kahan, nu[a], nw[a], d);
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < N; ++a)
+                next_node(D, kahan, edge_index(ex * N + a, iz, nx, nz, N), m[a], f[a], uu[a], wu[a], pv[a], pw[a],
+                          nu[a], nw[a]);
+        }
+        if (last_x)
+            next_node(D, kahan, edge_index(nx - 1, iz, nx, nz, N), 0.f, f[N], uu[N], wu[N], pv[N], pw[N], nu[N], nw[N]);
+    };
+
+    /* assembles a row force: z from the carried top row of the element
+       below (block CF), x from the element to the left */
+    auto assemble = [&](float (&f)[NP][3], float *CF) {
+        if (!valid) {
+#pragma unroll
+            for (int a = 0; a < NP; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) f[a][k] = 0.f;
+        }
+        if (b == 0) {
+            float cf[RL::CFS];
+#pragma unroll
+            for (int v = 0; v < (3 * NP + 3) / 4; ++v) {
+                const float4 t4 = *reinterpret_cast<const float4 *>(CF + 4 * v);
+                cf[4 * v] = t4.x; cf[4 * v + 1] = t4.y; cf[4 * v + 2] = t4.z; cf[4 * v + 3] = t4.w;
+            }
+#pragma unroll
+            for (int a = 0; a < NP; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) f[a][k] += cf[a * 3 + k];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (act && b == N) {
+#pragma unroll
+            for (int a = 0; a < NP; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) CF[a * 3 + k] = f[a][k];
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float s = __shfl(f[N][k], lane - NP, 64);
+            if (e > 0) f[0][k] += s;
+        }
+    };
+
+    auto row_material = [&](int ez, int rid, float (&f)[NP][3], const float (&u)[NP][3]) {
+        const long rowoff = (long)(ez * N + b) * ld + (long)exl * N;
+        const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
+        if constexpr (!MIXED) {
+            const SEMSKS_GMEM float *r = gp(D.mat) + (long)rid * SEMSKS_MAT_STRIDE;
+            float C[15];
+#pragma unroll
+            for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
+            row_force<N, true, ORTHO>(u, f, b, rc, Ub, Qb, C, gp(D.mat), idrow, act);
+        } else {
+            row_force<N, false, ORTHO>(u, f, b, rc, Ub, Qb, nullptr, gp(D.mat), idrow, act);
+        }
+    };
+
+    if (act && b == N) {
+#pragma unroll
+        for (int i = 0; i < RL::CFS; ++i) { CF1[i] = 0.f; CF2[i] = 0.f; }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    RowFetch<N> cur, prv;
+    float v1[NP][3], v1p[NP][3];
+    uint32_t w1[NP], w1p[NP];
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+        w1[a] = w1p[a] = 0u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v1[a][k] = v1p[a][k] = 0.f;
+    }
+    fetch(r0, cur);
+    prv = cur;
+    for (int ez = r0; ez < rend; ++ez) {
+        RowFetch<N> nxt;
+        if (ez < r1) {
+            /* ---- step 1 on element row ez ---- */
+            const int iz = ez * N + b;
+            const bool top = ez == nez - 1;
+            float u[NP][3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                for (int a = 0; a < N; ++a) u[a][k] = cur.u[a][k];
+                const float nb = __shfl(cur.u[0][k], lane + NP, 64);
+                u[N][k] = direct ? cur.un[k] : nb;
+            }
+            float f[NP][3];
+#pragma unroll
+            for (int a = 0; a < NP; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) f[a][k] = 0.f;
+            fetch_prev(ez, cur);
+            row_material(ez, cur.rid, f, u);
+            assemble(f, CF1);
+            if (valid && (b < N || top)) {
+                float pv[NP][3];
+                uint32_t pw[NP], wu[NP];
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    pv[a][0] = cur.pv[a].x; pv[a][1] = cur.pv[a].y; pv[a][2] = cur.pv[a].z;
+                    pw[a] = __float_as_uint(cur.pv[a].w);
+                    wu[a] = cur.c[a];
+                }
+                wu[N] = cur.cn;
+                pv[N][0] = pv[N][1] = pv[N][2] = 0.f;
+                pw[N] = 0u;
+                if (last_x) {
+                    const float4 t = *reinterpret_cast<const SEMSKS_GMEM float4 *>(UP + 4 * ((long)iz * ld + nx - 1));
+                    pv[N][0] = t.x; pv[N][1] = t.y; pv[N][2] = t.z; pw[N] = __float_as_uint(t.w);
+                }
+                row_next(iz, f, u, wu, pv, pw, cur.m, v1, w1);
+                if (inj && (iz == nbl || (ex * N <= nbl && nbl <= ex * N + N))) {
+#pragma unroll
+                    for (int a = 0; a < NP; ++a) {
+                        const int ix = ex * N + a;
+                        if ((a < N || last_x) && is_inject_node(ix, iz, nbl)) {
+                            float d[3];
+                            inject_delta(D, it + 1, ix, iz, u[a][0], u[a][2], d);
+                            kahan3(kahan, v1[a], w1[a], d);
+                        }
+                    }
+                }
+                if (own_x && ez >= z0 && ez < z0 + rows) {
+                    SEMSKS_GMEM float4 *o4 = reinterpret_cast<SEMSKS_GMEM float4 *>(U1 + 4 * ((long)iz * ld + (long)ex * N));
+#pragma unroll
+                    for (int a = 0; a < N; ++a) o4[a] = make_float4(v1[a][0], v1[a][1], v1[a][2], __uint_as_float(w1[a]));
+                    if (last_x) o4[N] = make_float4(v1[N][0], v1[N][1], v1[N][2], __uint_as_float(w1[N]));
+                }
+            }
+            /* the next row's loads go out before the second step */
+            __builtin_amdgcn_sched_barrier(0);
+            if (ez + 1 < r1) fetch(ez + 1, nxt);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (two && ez - 1 >= rs2) {
+            /* ---- step 2 on element row ez-1 ---- */
+            const int r = ez - 1;
+            const bool topr = r == nez - 1;
+            const bool mine = b < N || topr;          /* u(t+dt) of the lane's own row */
+            float u2[NP][3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    const float s = __shfl(v1[a][k], lane - N, 64);
+                    u2[a][k] = mine ? v1p[a][k] : s;
+                }
+                const float sn = __shfl(v1[N][k], lane - N, 64);
+                const float dn = mine ? v1p[N][k] : sn;
+                const float nb = __shfl(u2[0][k], lane + NP, 64);
+                u2[N][k] = direct ? dn : nb;
+            }
+            float f[NP][3];
+#pragma unroll
+            for (int a = 0; a < NP; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) f[a][k] = 0.f;
+            row_material(r, prv.rid, f, u2);
+            assemble(f, CF2);
+            if (own_x && mine && r >= z0 && r < z0 + rows) {
+                const int iz = r * N + b;
+                float pv[NP][3], nu[NP][3];
+                uint32_t pw[NP], nw[NP];
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    pv[a][0] = prv.u[a][0]; pv[a][1] = prv.u[a][1]; pv[a][2] = prv.u[a][2];
+                    pw[a] = prv.c[a];
+                }
+                pv[N][0] = prv.un[0]; pv[N][1] = prv.un[1]; pv[N][2] = prv.un[2];
+                pw[N] = prv.cn;
+                row_next(iz, f, v1p, w1p, pv, pw, prv.m, nu, nw);
+                SEMSKS_GMEM float4 *o4 = reinterpret_cast<SEMSKS_GMEM float4 *>(U2 + 4 * ((long)iz * ld + (long)ex * N));
+#pragma unroll
+                for (int a = 0; a < N; ++a) o4[a] = make_float4(nu[a][0], nu[a][1], nu[a][2], __uint_as_float(nw[a]));
+                if (last_x) o4[N] = make_float4(nu[N][0], nu[N][1], nu[N][2], __uint_as_float(nw[N]));
+            }
+        }
+        prv = cur;
+#pragma unroll
+        for (int a = 0; a < NP; ++a) {
+            w1p[a] = w1[a];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v1p[a][k] = v1[a][k];
+        }
+        if (ez + 1 < r1) cur = nxt;
+    }
+}
+
+/* persistent two-step sweep over the tile list (hpn = 4 tiles only); an
+   instance with one step left takes only the first step; smax = 1 makes
+   every tile take one step */
+template <int DUMMY>
+__global__ __launch_bounds__(256, 2) void step2_kernel(const InstDesc *__restrict__ insts, const int *__restrict__ levels,
+                                                       const int4 *__restrict__ tiles, const int *__restrict__ ntiles, int smax)
+{
+    __shared__ float lds_blocks[4][RowLayout<4>::WAVE_FLOATS + RowLayout<4>::NE * RowLayout<4>::CFS];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int nt = *ntiles;
+    for (int t = wave; t < nt; t += nwaves) {
+        int4 tile = tiles[t];
+        tile.x = __builtin_amdgcn_readfirstlane(tile.x);
+        tile.y = __builtin_amdgcn_readfirstlane(tile.y);
+        tile.z = __builtin_amdgcn_readfirstlane(tile.z);
+        tile.w = __builtin_amdgcn_readfirstlane(tile.w);
+        const InstDesc &D = insts[tile.x];
+        if (!D.active || D.hpn != 4) continue;
+        const int it = levels[tile.x];
+        if (it >= D.nst - 1) continue;
+        const bool two = smax >= 2 && it + 1 < D.nst - 1;
+        float *lds = lds_blocks[threadIdx.x >> 6];
+        const int rows = tile.w & 0xFFFF;
+        if (D.ortho) {
+            if (tile.w >> 16) process_tile2<true, true>(D, it & 3, it, two, tile.y, tile.z, rows, lane, lds);
+            else process_tile2<false, true>(D, it & 3, it, two, tile.y, tile.z, rows, lane, lds);
+        } else {
+            if (tile.w >> 16) process_tile2<true, false>(D, it & 3, it, two, tile.y, tile.z, rows, lane, lds);
+            else process_tile2<false, false>(D, it & 3, it, two, tile.y, tile.z, rows, lane, lds);
+        }
     }
 }
 
@@ -1084,6 +1484,7 @@ struct Slot {
     size_t bytes = 0;
     std::vector<void *> allocs;       /* owned device allocations */
     std::vector<uint16_t> emat;       /* host copy of the element-uniform record ids (owner) */
+    bool two_ok = false;              /* may take two steps per sweep */
 };
 
 /* A tile may run the single-record kernel when each of its element rows
@@ -1096,6 +1497,24 @@ bool tile_single_record(const Slot &o, int n, int x0, int z0, int rows)
     const int ex1 = std::min(nex, x0 - 1 + 64 / (n + 1));
     const int ez0 = z0 > 0 ? z0 - 1 : 0;
     const int ez1 = std::min(nez, z0 + rows);
+    for (int ez = ez0; ez < ez1; ++ez) {
+        const uint16_t r = o.emat[(size_t)ez * nex + ex0];
+        if (r == 0xFFFF) return false;
+        for (int ex = ex0 + 1; ex < ex1; ++ex)
+            if (o.emat[(size_t)ez * nex + ex] != r) return false;
+    }
+    return true;
+}
+
+/* the same for a two-step tile: element columns x0-2..x0+9 and rows
+   z0-2..z0+rows */
+bool tile2_single_record(const Slot &o, int x0, int z0, int rows)
+{
+    const int nex = o.meta.nex, nez = o.meta.nez;
+    const int ex0 = x0 >= 2 ? x0 - 2 : 0;
+    const int ex1 = std::min(nex, x0 - 2 + 64 / 5);
+    const int ez0 = z0 >= 2 ? z0 - 2 : 0;
+    const int ez1 = std::min(nez, z0 + rows + 1);
     for (int ez = ez0; ez < ez1; ++ez) {
         const uint16_t r = o.emat[(size_t)ez * nex + ex0];
         if (r == 0xFFFF) return false;
@@ -1150,6 +1569,9 @@ struct semsks_engine {
     int graph_steps = 0;
     unsigned hpn_mask = 0, graph_mask = 0;
     bool any_sponge = false, graph_sponge = false;
+    /* two steps per sweep (step2_kernel) for the current instance set, and
+       what the captured graph was built for */
+    bool two = false, graph_two = false;
     int graph_off[SEMSKS_MAXNP] = {0};  /* tile offsets the captured step was built with */
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> tev;      /* per-step timing events */
@@ -1197,6 +1619,17 @@ static semsks_status upload_tables(semsks_engine *e)
     HIPCHK(hipMemcpyAsync(e->d_desc, h.data(), h.size() * sizeof(InstDesc), hipMemcpyHostToDevice, e->stream));
     e->any_sponge = false;
     for (auto &s : e->slots) e->any_sponge |= s.used && s.d.atten > 0;
+    /* two steps per sweep when asked for and every instance allows it:
+       hpn = 4, no sponge, no receiver on an injection node (its trace of
+       the first level would see that level after the injection of the next
+       step) */
+    bool two = e->opts.steps_per_sweep == 2, any_used = false;
+    for (auto &s : e->slots)
+        if (s.used) { any_used = true; two = two && s.two_ok && s.meta.hpn == 4 && s.d.atten <= 0; }
+    if (e->two != (two && any_used)) {
+        e->two = two && any_used;
+        e->tuned = false;
+    }
     /* tiles, grouped by hpn, mixed (per-node material) tiles first, slots
        interleaved per spatial tile so that concurrent waves touch the same
        region of a shared mesh */
@@ -1209,16 +1642,19 @@ static semsks_status upload_tables(semsks_engine *e)
         int maxx = 0, maxz = 0;
         for (auto &s : e->slots)
             if (s.used && s.meta.hpn == n) { maxx = std::max(maxx, s.meta.nex); maxz = std::max(maxz, s.meta.nez); }
+        const int xc = e->two ? XC2 : xchunk_of(n);
         for (int mixed = 1; mixed >= 0; --mixed)
             for (int z0 = 0; z0 < maxz; z0 += tzg)
-                for (int x0 = 0; x0 < maxx; x0 += xchunk_of(n))
+                for (int x0 = 0; x0 < maxx; x0 += xc)
                     for (size_t k = 0; k < e->slots.size(); ++k) {
                         const Slot &s = e->slots[k];
                         if (!s.used || s.meta.hpn != n) continue;
                         if (z0 >= s.meta.nez || x0 >= s.meta.nex) continue;
                         const Slot &o = s.share >= 0 ? e->slots[s.share] : s;
                         const int rows = std::min(tzg, s.meta.nez - z0);
-                        if (tile_single_record(o, n, x0, z0, rows) == (mixed != 0)) continue;
+                        const bool single = e->two ? tile2_single_record(o, x0, z0, rows)
+                                                   : tile_single_record(o, n, x0, z0, rows);
+                        if (single == (mixed != 0)) continue;
                         t.push_back(make_int4((int)k, x0, z0, rows | (mixed << 16)));
                     }
         cnt[n] = (int)t.size() - e->tile_off[n];
@@ -1237,7 +1673,7 @@ static semsks_status upload_tables(semsks_engine *e)
     HIPCHK(hipStreamSynchronize(e->stream));
     e->dirty = false;
     if (e->opts.tile_z <= 0) {
-        auto it = e->tz_cache.find(tune_signature(e));
+        auto it = e->tz_cache.find(tune_signature(e) | ((uint64_t)e->two << 32));
         e->tuned = it != e->tz_cache.end();
         if (e->tuned && it->second != tzg) {
             e->tz_cur = it->second;
@@ -1285,17 +1721,33 @@ static void launch_tiles(const semsks_engine *e, int n, const InstDesc *desc, co
     }
 }
 
-static semsks_status launch_step(semsks_engine *e, hipEvent_t ta, hipEvent_t tb)
+/* one launch: steps every instance by up to smax levels (smax = 2 only in
+   two-step mode) */
+static semsks_status launch_step(semsks_engine *e, int smax, hipEvent_t ta, hipEvent_t tb)
 {
     const int nslots = (int)e->slots.size();
+    if (!e->two) smax = 1;
     hipLaunchKernelGGL(inject_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
     if (ta) HIPCHK(hipEventRecord(ta, e->stream));
-    for (int n = 3; n <= 6; ++n)
-        if (e->hpn_mask & (1u << n))
-            launch_tiles<false>(e, n, e->d_desc, e->d_levels, e->d_tiles + e->tile_off[n], e->d_ntiles + n);
+    if (e->two) {
+        if (e->hpn_mask & (1u << 4)) {
+            static int per_cu = 0;
+            if (per_cu < 1 &&
+                (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&step2_kernel<0>),
+                                                              256, 0) != hipSuccess || per_cu < 1))
+                per_cu = 2;
+            const int nb = e->opts.waves_per_cu > 0 ? std::max(1, e->n_cu * e->opts.waves_per_cu / 4) : e->n_cu * per_cu;
+            hipLaunchKernelGGL(step2_kernel<0>, dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels,
+                               e->d_tiles + e->tile_off[4], e->d_ntiles + 4, smax);
+        }
+    } else {
+        for (int n = 3; n <= 6; ++n)
+            if (e->hpn_mask & (1u << n))
+                launch_tiles<false>(e, n, e->d_desc, e->d_levels, e->d_tiles + e->tile_off[n], e->d_ntiles + n);
+    }
     if (tb) HIPCHK(hipEventRecord(tb, e->stream));
     if (e->any_sponge) hipLaunchKernelGGL(sponge_kernel, dim3(64, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
-    hipLaunchKernelGGL(record_kernel, dim3(nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
+    hipLaunchKernelGGL(record_kernel, dim3(nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels, smax);
     HIPCHK(hipGetLastError());
     return SEMSKS_OK;
 }
@@ -1311,8 +1763,9 @@ static semsks_status build_graph(semsks_engine *e, int steps)
 {
     drop_graph(e);
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < steps; ++s) {
-        semsks_status st = launch_step(e, nullptr, nullptr);
+    const int S = e->two ? 2 : 1;
+    for (int s = 0; s < steps; s += S) {
+        semsks_status st = launch_step(e, S, nullptr, nullptr);
         if (st != SEMSKS_OK) { hipGraph_t g; hipStreamEndCapture(e->stream, &g); if (g) hipGraphDestroy(g); return st; }
     }
     HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
@@ -1320,6 +1773,7 @@ static semsks_status build_graph(semsks_engine *e, int steps)
     e->graph_steps = steps;
     e->graph_mask = e->hpn_mask;
     e->graph_sponge = e->any_sponge;
+    e->graph_two = e->two;
     memcpy(e->graph_off, e->tile_off, sizeof e->graph_off);
     return SEMSKS_OK;
 }
@@ -1391,7 +1845,7 @@ static int nt_sem_of(const semsks_problem *p) { return (int)((float)((p->nt - 3)
 size_t semsks_footprint(const semsks_problem *p, int shared)
 {
     const size_t nodes = (size_t)ld_of(p) * nz_of(p);
-    size_t b = nodes * 4 * (3 * 4);                 /* three levels of {x, y, z, packed compensation} */
+    size_t b = nodes * 4 * (4 * 4);                 /* four levels of {x, y, z, packed compensation} */
     if (!shared) b += nodes * 2 + (size_t)p->nex * p->nez * 2 +
                       (size_t)p->n_mat * (64 + 4 * (p->hpn + 1) * (p->hpn + 1)) +
                       (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24;
@@ -1468,11 +1922,19 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     s.nt_sem = nt_sem_of(p);
     s.nlev = std::max(1, std::min(s.nt_sem, p->nst));
     s.share = share;
+    s.two_ok = true;
+    for (int i = 0; i < p->ntrace; ++i)
+        if (((p->rec_iz[i] == p->neh * p->hpn && p->rec_ix[i] >= p->neh * p->hpn) ||
+             (p->rec_ix[i] == p->neh * p->hpn && p->rec_iz[i] > p->neh * p->hpn)))
+            s.two_ok = false;
     InstDesc &d = s.d;
     memset(&d, 0, sizeof d);
     d.nex = p->nex; d.nez = p->nez; d.nx = s.nx; d.nz = s.nz; d.ld = s.ld;
     d.hpn = p->hpn; d.neh = p->neh; d.nst = p->nst; d.nt_sem = s.nt_sem; d.solver = p->solver;
     d.kahan = (p->flags & SEMSKS_F_NO_KAHAN) ? 0 : 1;
+    d.ortho = 1;     /* every record orthotropic in the mesh axes: the short stress form */
+    for (int i = 0; i < p->n_mat; ++i)
+        if (!semsks_ortho_record(p->mat_table + (size_t)i * SEMSKS_MAT_STRIDE + 1)) d.ortho = 0;
     d.ntrace = p->ntrace;
     const size_t nodes = (size_t)s.ld * s.nz;
     const int mt = 2 * (s.nx + s.nz), sht = s.nx + s.nz;
@@ -1480,10 +1942,10 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
 #define ACHK(x) do { if ((x) != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "device allocation failed"); goto fail; } } while (0)
     {
         float *st_all = nullptr;
-        const int nplanes = 3 * 4;                   /* three levels of 16-byte records */
+        const int nplanes = 4 * 4;                   /* four levels of 16-byte records */
         ACHK(dalloc(s, &st_all, nodes * nplanes));
         ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->stream));
-        for (int l = 0; l < 3; ++l) d.u[l] = st_all + 4 * l * nodes;
+        for (int l = 0; l < 4; ++l) d.u[l] = st_all + 4 * l * nodes;
     }
     if (p->atten > 0) {   /* atten_taper (sem2d_psvsh.c:1725-1738), double then float */
         const int nh = p->neh * p->hpn;
@@ -1674,9 +2136,11 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
        spare runs two real steps at each candidate tile height and keeps the
        fastest (the tiling does not change the arithmetic, so these steps
        are ordinary steps of the run) */
-    static const int cand[] = {4, 8, 16, 32};
-    const int ncand = (int)(sizeof cand / sizeof cand[0]);
-    if (e->opts.tile_z <= 0 && !e->tuned && nsteps >= 4 * ncand && sync != 2) {
+    static const int cand1[] = {4, 8, 16, 32}, cand2[] = {8, 16, 32, 64};
+    const int *cand = e->two ? cand2 : cand1;
+    const int ncand = 4;
+    const int S = e->two ? 2 : 1;     /* steps per launch */
+    if (e->opts.tile_z <= 0 && !e->tuned && nsteps >= 4 * ncand * S && sync != 2) {
         while ((int)e->tev.size() < 4) {
             hipEvent_t v;
             HIPCHK(hipEventCreate(&v));
@@ -1689,19 +2153,19 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
             semsks_status st = upload_tables(e);
             if (st) return st;
             for (int r = 0; r < 2; ++r) {
-                st = launch_step(e, e->tev[2 * r], e->tev[2 * r + 1]);
+                st = launch_step(e, S, e->tev[2 * r], e->tev[2 * r + 1]);
                 if (st) return st;
             }
             HIPCHK(hipEventSynchronize(e->tev[3]));
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, e->tev[2], e->tev[3]));
             if (ms < best_ms) { best_ms = ms; best = cand[c]; }
-            done += 2;
+            done += 2 * S;
         }
         e->tz_cur = best;
         semsks_status st = upload_tables(e);
         if (st) return st;
-        e->tz_cache[tune_signature(e)] = best;
+        e->tz_cache[tune_signature(e) | ((uint64_t)e->two << 32)] = best;
         e->tuned = true;
         drop_graph(e);
     }
@@ -1709,26 +2173,36 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
         const int G = 16;
         if (nsteps - done >= G) {
             if (!e->gexec || e->graph_steps != G || e->graph_mask != e->hpn_mask ||
-                memcmp(e->graph_off, e->tile_off, sizeof e->graph_off) != 0 || e->graph_sponge != e->any_sponge) {
+                memcmp(e->graph_off, e->tile_off, sizeof e->graph_off) != 0 || e->graph_sponge != e->any_sponge ||
+                e->graph_two != e->two) {
                 semsks_status st = build_graph(e, G);
                 if (st) return st;
             }
             for (; done + G <= nsteps; done += G) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
         }
-        for (; done < nsteps; ++done) { semsks_status st = launch_step(e, nullptr, nullptr); if (st) return st; }
+        while (done < nsteps) {
+            const int s = std::min(S, nsteps - done);
+            semsks_status st = launch_step(e, s, nullptr, nullptr);
+            if (st) return st;
+            done += s;
+        }
     } else {
         /* eager launches with an event pair around every step-kernel launch
-           (the first 1024 steps of the call are timed) */
-        const int ntimed = sync ? std::min(nsteps - done, 1024) : 0;
+           (the launches of the first 1024 steps of the call are timed) */
+        const int nl = (nsteps - done + S - 1) / S;
+        const int ntimed = sync ? std::min(nl, 1024 / S) : 0;
         while ((int)e->tev.size() < 2 * ntimed) {
             hipEvent_t v;
             HIPCHK(hipEventCreate(&v));
             e->tev.push_back(v);
         }
-        for (int i = done; i < nsteps; ++i) {
-            const int j = i - done;
-            semsks_status st = j < ntimed ? launch_step(e, e->tev[2 * j], e->tev[2 * j + 1]) : launch_step(e, nullptr, nullptr);
+        int tsteps = 0;
+        for (int j = 0; done < nsteps; ++j) {
+            const int s = std::min(S, nsteps - done);
+            semsks_status st = j < ntimed ? launch_step(e, s, e->tev[2 * j], e->tev[2 * j + 1]) : launch_step(e, s, nullptr, nullptr);
             if (st) return st;
+            if (j < ntimed) tsteps += s;
+            done += s;
         }
         if (ntimed) {
             HIPCHK(hipEventSynchronize(e->tev[2 * ntimed - 1]));
@@ -1737,7 +2211,7 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
                 HIPCHK(hipEventElapsedTime(&ms, e->tev[2 * i], e->tev[2 * i + 1]));
                 kern_ms += ms;
             }
-            e->timed_steps = ntimed;
+            e->timed_steps = tsteps;
         }
     }
     HIPCHK(hipEventRecord(e->ev[1], e->stream));
@@ -1766,6 +2240,11 @@ semsks_status semsks_sync(semsks_engine *e)
 {
     HIPCHK(hipStreamSynchronize(e->stream));
     return SEMSKS_OK;
+}
+
+int semsks_sweep_steps(semsks_engine *e)
+{
+    return e && e->two ? 2 : 1;
 }
 
 semsks_status semsks_slot_status(semsks_engine *e, int k, int *level, int *done)
@@ -1798,7 +2277,7 @@ semsks_status semsks_get_field(semsks_engine *e, int k, float *out)
 {
     if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
     const Slot &s = e->slots[k];
-    const int p = e->h_levels[k] % 3;
+    const int p = e->h_levels[k] & 3;
     std::vector<float> aos((size_t)s.ld * s.nz * 4);
     HIPCHK(hipMemcpyAsync(aos.data(), s.d.u[p], aos.size() * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));

@@ -29,10 +29,10 @@
 struct InstDesc {
     int nex, nez, nx, nz, ld;
     int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active;
-    /* displacement of three time levels, level it in u[it % 3]; one
+    /* displacement of four time levels, level it in u[it % 4]; one
        16-byte record per node: x, y, z and the packed Kahan compensation
        of the three components (bits): node q at u[l][4q..4q+3] */
-    float *u[3];
+    float *u[4];
     float *fout[3];                 /* force-only mode output planes, else null */
     const uint16_t *mat_id;
     const uint16_t *emat;
@@ -51,7 +51,7 @@ struct InstDesc {
     float Dg[SEMSKS_MAXNP][SEMSKS_MAXNP];
     float Dw[SEMSKS_MAXNP][SEMSKS_MAXNP];
     float cw[SEMSKS_MAXNP];
-    int pad[3];
+    int ortho;                      /* all material records orthotropic in the x, y, z axes */
 };
 
 #if defined(__HIPCC__)

@@ -29,7 +29,7 @@ EXPORTS = [
     "semsks_model_free", "semsks_write_seis", "semsks_last_error", "semsks_engine_create",
     "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_advance",
     "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_retire", "semsks_sync",
-    "semsks_last_timing", "semsks_last_timed_steps", "semsks_run_problem", "semsks_stiff",
+    "semsks_last_timing", "semsks_last_timed_steps", "semsks_sweep_steps", "semsks_run_problem", "semsks_stiff",
     "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve", "semsks_uvw2sac",
 ]
 
@@ -66,7 +66,7 @@ class ServeStats(C.Structure):
 
 class EngineOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_slots", C.c_int), ("tile_z", C.c_int),
-                ("waves_per_cu", C.c_int), ("use_graph", C.c_int)]
+                ("waves_per_cu", C.c_int), ("use_graph", C.c_int), ("steps_per_sweep", C.c_int)]
 
 
 _lib = None
@@ -103,6 +103,8 @@ def lib() -> C.CDLL:
     L.semsks_last_timing.argtypes = [vp, C.POINTER(C.c_double)]
     L.semsks_last_timed_steps.restype = C.c_int
     L.semsks_last_timed_steps.argtypes = [vp]
+    L.semsks_sweep_steps.restype = C.c_int
+    L.semsks_sweep_steps.argtypes = [vp]
     L.semsks_run_problem.argtypes = [C.POINTER(Problem), vp]
     L.semsks_stiff.argtypes = [vp, C.c_int, vp, vp]
     L.semsks_get_setup.argtypes = [vp, C.c_int, vp, vp, vp]
@@ -190,9 +192,9 @@ def footprint(p: Problem, shared: bool = False) -> int:
 
 
 class Engine:
-    def __init__(self, max_slots=1, hbm_budget=0, tile_z=0, waves_per_cu=0, use_graph=1):
+    def __init__(self, max_slots=1, hbm_budget=0, tile_z=0, waves_per_cu=0, use_graph=1, steps_per_sweep=0):
         o = EngineOpts(hbm_budget=hbm_budget, max_slots=max_slots, tile_z=tile_z,
-                       waves_per_cu=waves_per_cu, use_graph=use_graph)
+                       waves_per_cu=waves_per_cu, use_graph=use_graph, steps_per_sweep=steps_per_sweep)
         h = C.c_void_p()
         _check(lib().semsks_engine_create(C.byref(o), C.byref(h)))
         self._h = h
@@ -295,6 +297,11 @@ class Engine:
         t = (C.c_double * 2)()
         lib().semsks_last_timing(self._h, t)
         return t[0], t[1], lib().semsks_last_timed_steps(self._h)
+
+    def sweep_steps(self) -> int:
+        """Time steps per step-kernel launch for the current instances (2 in
+        the two-step sweep, else 1)."""
+        return int(lib().semsks_sweep_steps(self._h))
 
     def retire(self, slot: int):
         _check(lib().semsks_retire(self._h, slot))
