@@ -165,7 +165,9 @@ void semsks_engine_destroy(semsks_engine *e);
    footprint does not fit the budget. */
 semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share_slot, int *slot);
 
-/* Device bytes one problem needs (for admission control). */
+/* Device bytes one problem needs (for admission control) in an engine
+   that takes one step per launch; one with steps_per_sweep = 2 keeps a
+   fourth time level, 16 more bytes per (row-padded) node. */
 size_t semsks_footprint(const semsks_problem *p, int shared_material);
 
 /* Advances every active instance by up to nsteps time steps (the loop body

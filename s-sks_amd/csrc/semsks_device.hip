@@ -43,12 +43,13 @@
  * one-step kernel already keeps the vector ALUs ~2/3 busy, so one step per
  * launch is the default.
  *
- * State in HBM: four time levels of u rotating (level it in buffer it % 4),
- * one 16-byte record per node {x, y, z, packed Kahan compensation}.  A step
- * reads level it (with halos) and level it-1 and writes level it+1 (and
- * it+2 in a two-step sweep, which therefore never overwrites a level that a
- * neighbouring tile may still read).  Rows are padded to a multiple of 64
- * nodes.
+ * State in HBM: three time levels of u rotating (level it in buffer
+ * it % 3), one 16-byte record per node {x, y, z, packed Kahan
+ * compensation}.  A step reads level it (with halos) and level it-1 and
+ * writes level it+1.  An engine that takes two steps per sweep keeps four
+ * levels (it % 4) and also writes it+2, which then never overwrites a level
+ * that a neighbouring tile may still read.  Rows are padded to a multiple
+ * of 64 nodes.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -435,10 +436,10 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels)
     const int it = levels[blockIdx.y];
     if (it >= D.nst - 1) return;
     if (it + 1 >= D.nt_sem) return;
-    const int p = it & 3, nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld;
+    const int L = D.nlev, p = it % L, nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld;
     const bool mp = D.solver == SEMSKS_SOLVER_PSV_MP;
     float *U = D.u[p];
-    const float *Up = D.u[(p + 3) & 3];
+    const float *Up = D.u[(p + L - 1) % L];
     const int nb = nx - nbl, nl = nz - nbl - 1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb + nl; i += gridDim.x * blockDim.x) {
         const int ix = i < nb ? nbl + i : nbl;
@@ -464,7 +465,7 @@ __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
     if (!D.active || D.atten <= 0) return;
     const int it = levels[blockIdx.y];
     if (it >= D.nst - 1) return;
-    const int pn = (it + 1) & 3, nh = D.nh, nx = D.nx, nz = D.nz, ld = D.ld, at = D.atten;
+    const int pn = (it + 1) % D.nlev, nh = D.nh, nx = D.nx, nz = D.nz, ld = D.ld, at = D.atten;
     const long nb = (long)nh * nx, ns = (long)(nz - nh) * nh, ntop = (long)nh * (nx - 2 * nh);
     const long total = nb + (at > 1 ? ns : 0) + (at > 2 ? ns : 0) + (at > 3 ? ntop : 0);
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
@@ -502,7 +503,7 @@ __global__ void record_kernel(const InstDesc *insts, int *levels, int smax)
     if (it >= D.nst - 1) return;
     const int adv = min(smax, D.nst - 1 - it), nt = D.ntrace;
     for (int l = 1; l <= adv; ++l) {
-        const int lev = it + l, p = lev & 3;
+        const int lev = it + l, p = lev % D.nlev;
         float *tr = D.traces + (long)lev * 3 * nt;
         for (int r = threadIdx.x; r < nt; r += blockDim.x) {
             long q = D.rec[r];
@@ -682,13 +683,13 @@ __device__ __forceinline__ void update_one(const InstDesc &D, int p, long q, int
                                            const float (&f)[3], const float (&uc)[3], uint32_t w)
 {
     const int e = edge_index(ix, iz, D.nx, D.nz, D.hpn);
-    const float4 pv = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.u[(p + 3) & 3]) + 4 * q);
+    const float4 pv = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.u[(p + D.nlev - 1) % D.nlev]) + 4 * q);
     const float up[3] = {pv.x, pv.y, pv.z};
     const float m = e < 0 ? mdt2_of<N>(D, mid, ix, iz) : 0.f;
     float nu[3];
     uint32_t wn;
     next_node(D, D.kahan != 0, e, m, f, uc, w, up, __float_as_uint(pv.w), nu, wn);
-    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[(p + 1) & 3]) + 4 * q) = make_float4(nu[0], nu[1], nu[2], __uint_as_float(wn));
+    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[(p + 1) % D.nlev]) + 4 * q) = make_float4(nu[0], nu[1], nu[2], __uint_as_float(wn));
 }
 
 /* Lane layout of the element-force/update kernel: a 64-lane wave holds
@@ -876,7 +877,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     const RowCoef<N> rc = row_coef<N>(b);
 
     const SEMSKS_GMEM float *U = gp(D.u[p]);
-    const int pp = (p + 3) & 3, pn = (p + 1) & 3;       /* buffers of levels t-dt, t+dt */
+    const int pp = (p + D.nlev - 1) % D.nlev, pn = (p + 1) % D.nlev;   /* buffers of levels t-dt, t+dt */
 
     const int ez0 = z0 > 0 ? z0 - 1 : 0;
     const int ez1 = z0 + rows;
@@ -898,7 +899,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
             r.un[0] = t.x; r.un[1] = t.y; r.un[2] = t.z; r.cn = __float_as_uint(t.w);
         }
         r.rid = 0;
-        if constexpr (!MIXED) r.rid = __builtin_amdgcn_readfirstlane((int)gp(D.emat)[(long)ez * nex + ex_first]);
+        if constexpr (!MIXED) r.rid = (int)cp(D.emat)[(long)ez * nex + ex_first];
         if constexpr (N == 4 && !FORCE_ONLY) {
             const bool own = own_x && ez >= z0 && b < N;
             const bool vec = own && iz != 0 && ex != 0;
@@ -957,7 +958,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
 #pragma unroll
             for (int k = 0; k < 3; ++k) f[a][k] = 0.f;
         if constexpr (!MIXED) {
-            const SEMSKS_GMEM float *r = gp(D.mat) + (long)rid * SEMSKS_MAT_STRIDE;
+            const SEMSKS_CMEM float *r = cp(D.mat) + (long)rid * SEMSKS_MAT_STRIDE;
             float C[15];
 #pragma unroll
             for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
@@ -1078,7 +1079,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : 2) void step_kernel(const InstDes
         if constexpr (!FORCE_ONLY) {
             const int it = levels[tile.x];
             if (it >= D.nst - 1) continue;
-            p = it & 3;
+            p = it % D.nlev;
         }
         float *lds = lds_blocks[threadIdx.x >> 6];
         const int rows = tile.w & 0xFFFF;
@@ -1135,9 +1136,9 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     const bool inj = two && it + 2 < D.nt_sem;   /* inject_kernel's test for level it+1 */
 
     const SEMSKS_GMEM float *U = gp(D.u[p]);
-    const SEMSKS_GMEM float *UP = gp(D.u[(p + 3) & 3]);
-    SEMSKS_GMEM float *U1 = gp(D.u[(p + 1) & 3]);
-    SEMSKS_GMEM float *U2 = gp(D.u[(p + 2) & 3]);
+    const SEMSKS_GMEM float *UP = gp(D.u[(p + 3) % 4]);
+    SEMSKS_GMEM float *U1 = gp(D.u[(p + 1) % 4]);
+    SEMSKS_GMEM float *U2 = gp(D.u[(p + 2) % 4]);
 
     const int r0 = z0 >= 2 ? z0 - 2 : 0;            /* first row of step 1 */
     const int rs2 = z0 >= 1 ? z0 - 1 : 0;           /* first row of step 2 */
@@ -1160,7 +1161,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
             r.un[0] = t.x; r.un[1] = t.y; r.un[2] = t.z; r.cn = __float_as_uint(t.w);
         }
         r.rid = 0;
-        if constexpr (!MIXED) r.rid = __builtin_amdgcn_readfirstlane((int)gp(D.emat)[(long)ez * nex + ex_first]);
+        if constexpr (!MIXED) r.rid = (int)cp(D.emat)[(long)ez * nex + ex_first];
     };
     /* u(t-dt) and dt^2/M of the row, loaded at the top of its first step */
     auto fetch_prev = [&](int ez, RowFetch<N> &r) {
@@ -1251,7 +1252,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
         const long rowoff = (long)(ez * N + b) * ld + (long)exl * N;
         const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
         if constexpr (!MIXED) {
-            const SEMSKS_GMEM float *r = gp(D.mat) + (long)rid * SEMSKS_MAT_STRIDE;
+            const SEMSKS_CMEM float *r = cp(D.mat) + (long)rid * SEMSKS_MAT_STRIDE;
             float C[15];
 #pragma unroll
             for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
@@ -1420,11 +1421,11 @@ __global__ __launch_bounds__(256, 2) void step2_kernel(const InstDesc *__restric
         float *lds = lds_blocks[threadIdx.x >> 6];
         const int rows = tile.w & 0xFFFF;
         if (D.ortho) {
-            if (tile.w >> 16) process_tile2<true, true>(D, it & 3, it, two, tile.y, tile.z, rows, lane, lds);
-            else process_tile2<false, true>(D, it & 3, it, two, tile.y, tile.z, rows, lane, lds);
+            if (tile.w >> 16) process_tile2<true, true>(D, it % 4, it, two, tile.y, tile.z, rows, lane, lds);
+            else process_tile2<false, true>(D, it % 4, it, two, tile.y, tile.z, rows, lane, lds);
         } else {
-            if (tile.w >> 16) process_tile2<true, false>(D, it & 3, it, two, tile.y, tile.z, rows, lane, lds);
-            else process_tile2<false, false>(D, it & 3, it, two, tile.y, tile.z, rows, lane, lds);
+            if (tile.w >> 16) process_tile2<true, false>(D, it % 4, it, two, tile.y, tile.z, rows, lane, lds);
+            else process_tile2<false, false>(D, it % 4, it, two, tile.y, tile.z, rows, lane, lds);
         }
     }
 }
@@ -1625,7 +1626,7 @@ static semsks_status upload_tables(semsks_engine *e)
        step) */
     bool two = e->opts.steps_per_sweep == 2, any_used = false;
     for (auto &s : e->slots)
-        if (s.used) { any_used = true; two = two && s.two_ok && s.meta.hpn == 4 && s.d.atten <= 0; }
+        if (s.used) { any_used = true; two = two && s.two_ok && s.meta.hpn == 4 && s.d.atten <= 0 && s.d.nlev == 4; }
     if (e->two != (two && any_used)) {
         e->two = two && any_used;
         e->tuned = false;
@@ -1845,7 +1846,7 @@ static int nt_sem_of(const semsks_problem *p) { return (int)((float)((p->nt - 3)
 size_t semsks_footprint(const semsks_problem *p, int shared)
 {
     const size_t nodes = (size_t)ld_of(p) * nz_of(p);
-    size_t b = nodes * 4 * (4 * 4);                 /* four levels of {x, y, z, packed compensation} */
+    size_t b = nodes * 4 * (3 * 4);                 /* three levels of {x, y, z, packed compensation} */
     if (!shared) b += nodes * 2 + (size_t)p->nex * p->nez * 2 +
                       (size_t)p->n_mat * (64 + 4 * (p->hpn + 1) * (p->hpn + 1)) +
                       (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24;
@@ -1906,7 +1907,8 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
             q.absorb != p->absorb)
             return semsks_set_error(SEMSKS_EINVAL, "share_slot %d has a different mesh", share);
     }
-    const size_t need = semsks_footprint(p, share >= 0);
+    const size_t need = semsks_footprint(p, share >= 0) +
+                        (e->opts.steps_per_sweep == 2 ? (size_t)ld_of(p) * nz_of(p) * 16 : 0);
     if (e->used + need > e->budget)
         return semsks_set_error(SEMSKS_ENOHBM, "slot needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
     for (int i = 0; i < p->ntrace; ++i)
@@ -1942,10 +1944,13 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
 #define ACHK(x) do { if ((x) != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "device allocation failed"); goto fail; } } while (0)
     {
         float *st_all = nullptr;
-        const int nplanes = 4 * 4;                   /* four levels of 16-byte records */
+        /* three levels of 16-byte records; four for an engine that may
+           take two steps per sweep */
+        d.nlev = e->opts.steps_per_sweep == 2 ? 4 : 3;
+        const int nplanes = d.nlev * 4;
         ACHK(dalloc(s, &st_all, nodes * nplanes));
         ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->stream));
-        for (int l = 0; l < 4; ++l) d.u[l] = st_all + 4 * l * nodes;
+        for (int l = 0; l < d.nlev; ++l) d.u[l] = st_all + 4 * l * nodes;
     }
     if (p->atten > 0) {   /* atten_taper (sem2d_psvsh.c:1725-1738), double then float */
         const int nh = p->neh * p->hpn;
@@ -2277,7 +2282,7 @@ semsks_status semsks_get_field(semsks_engine *e, int k, float *out)
 {
     if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
     const Slot &s = e->slots[k];
-    const int p = e->h_levels[k] & 3;
+    const int p = e->h_levels[k] % s.d.nlev;
     std::vector<float> aos((size_t)s.ld * s.nz * 4);
     HIPCHK(hipMemcpyAsync(aos.data(), s.d.u[p], aos.size() * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));

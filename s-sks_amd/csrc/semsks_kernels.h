@@ -28,10 +28,11 @@
 
 struct InstDesc {
     int nex, nez, nx, nz, ld;
-    int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active;
-    /* displacement of four time levels, level it in u[it % 4]; one
-       16-byte record per node: x, y, z and the packed Kahan compensation
-       of the three components (bits): node q at u[l][4q..4q+3] */
+    int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active, nlev;
+    /* displacement of nlev (3, or 4 for two-step sweeps) time levels,
+       level it in u[it % nlev]; one 16-byte record per node: x, y, z and
+       the packed Kahan compensation of the three components (bits): node q
+       at u[l][4q..4q+3] */
     float *u[4];
     float *fout[3];                 /* force-only mode output planes, else null */
     const uint16_t *mat_id;
@@ -68,6 +69,16 @@ template <class T>
 __device__ __forceinline__ SEMSKS_GMEM T *gp(T *p) { return (SEMSKS_GMEM T *)p; }
 template <class T>
 __device__ __forceinline__ const SEMSKS_GMEM T *gp(const T *p) { return (const SEMSKS_GMEM T *)p; }
+/* read-only data at a wave-uniform address (material records, element
+   record ids): the constant address space lets the compiler load it with
+   scalar loads into scalar registers */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SEMSKS_CMEM __attribute__((address_space(4)))
+#else
+#define SEMSKS_CMEM
+#endif
+template <class T>
+__device__ __forceinline__ const SEMSKS_CMEM T *cp(const T *p) { return (const SEMSKS_CMEM T *)p; }
 #endif
 
 /* edge-node numbering: bottom row iz=0 (nx) | left column iz>=1 (nz-1) |
