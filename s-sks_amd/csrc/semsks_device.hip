@@ -2138,15 +2138,16 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     e->timed_steps = 0;
     int done = 0;
     /* autotuner: with tile_z unset, the first call that has the steps to
-       spare runs two real steps at each candidate tile height and keeps the
-       fastest (the tiling does not change the arithmetic, so these steps
-       are ordinary steps of the run) */
+       spare runs three real launches at each candidate tile height and
+       keeps the height whose faster of the last two was fastest (the tiling
+       does not change the arithmetic, so these steps are ordinary steps of
+       the run) */
     static const int cand1[] = {4, 8, 16, 32}, cand2[] = {8, 16, 32, 64};
     const int *cand = e->two ? cand2 : cand1;
     const int ncand = 4;
     const int S = e->two ? 2 : 1;     /* steps per launch */
     if (e->opts.tile_z <= 0 && !e->tuned && nsteps >= 4 * ncand * S && sync != 2) {
-        while ((int)e->tev.size() < 4) {
+        while ((int)e->tev.size() < 6) {
             hipEvent_t v;
             HIPCHK(hipEventCreate(&v));
             e->tev.push_back(v);
@@ -2157,15 +2158,20 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
             e->tz_cur = cand[c];
             semsks_status st = upload_tables(e);
             if (st) return st;
-            for (int r = 0; r < 2; ++r) {
+            /* three launches; the faster of the last two counts */
+            for (int r = 0; r < 3; ++r) {
                 st = launch_step(e, S, e->tev[2 * r], e->tev[2 * r + 1]);
                 if (st) return st;
             }
-            HIPCHK(hipEventSynchronize(e->tev[3]));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e->tev[2], e->tev[3]));
+            HIPCHK(hipEventSynchronize(e->tev[5]));
+            float ms = 1e30f;
+            for (int r = 1; r < 3; ++r) {
+                float m = 0.f;
+                HIPCHK(hipEventElapsedTime(&m, e->tev[2 * r], e->tev[2 * r + 1]));
+                ms = std::min(ms, m);
+            }
             if (ms < best_ms) { best_ms = ms; best = cand[c]; }
-            done += 2 * S;
+            done += 3 * S;
         }
         e->tz_cur = best;
         semsks_status st = upload_tables(e);
