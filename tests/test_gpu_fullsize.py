@@ -1,0 +1,73 @@
+"""The full-size meshes of the benchmark (2000x500 elements, 16,010,001 GLL
+points per instance) through properties that do not depend on size, since
+the oracle cannot step them in seconds:
+
+  * tiling invariance: fixed tile heights, the autotuned one and the
+    two-step sweep give bit-identical fields;
+  * batching: an instance stepped with another one on the same mesh equals
+    its solo run, bit for bit;
+  * linearity: boundary records scaled by 2 (exact in FP32, and the
+    interpolation weights are exact at dp = sdt) give twice the field.
+
+The records start just before the incident pulse reaches the interface, so
+the fields carry the pulse after a few tens of steps.  One case is the
+tilted-axis anisotropic model with layer cuts inside tiles (general stress,
+per-node material tiles), the other the isotropic batch model (short
+orthotropic stress form)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import semsks
+from semsks import configs
+
+pytestmark = pytest.mark.gpu
+
+NST = 25
+
+
+def build(name):
+    case = configs.c2_case(nst=NST) if name == "c2" else configs.c3_case(nst=NST)
+    b = configs.Built(case)
+    pr = b.model.problem
+    k0 = max(0, int(1.2 / case.f0 / pr.sdt) - 8)       # pulse peak at t = 1.2/f0 at the first bottom node
+    with semsks.Engine() as e:
+        psv, sh = e.planewave(*b.planewave_args(k0 + NST + 3, p=0.05, psi=30.0), to_host=True)
+    return b, np.ascontiguousarray(psv[k0:]), np.ascontiguousarray(sh[k0:])
+
+
+def problem(b, psv, sh):
+    p = semsks.Problem.from_buffer_copy(b.model.problem)
+    p.nst, p.nt, p.dp = NST, psv.shape[0], p.sdt
+    p.psv = psv.ctypes.data_as(C.POINTER(C.c_float))
+    p.sh = sh.ctypes.data_as(C.POINTER(C.c_float))
+    return p
+
+
+def fields(problems, share=True, **eng):
+    with semsks.Engine(max_slots=len(problems), **eng) as e:
+        slots = []
+        for p in problems:
+            slots.append(e.admit(p, share=slots[0] if (share and slots) else -1))
+        e.advance(NST - 1)
+        return [e.field(s) for s in slots], e.sweep_steps()
+
+
+@pytest.mark.parametrize("name", ["c2", "c3"])
+def test_full_mesh_invariants(name):
+    print("case: %s full mesh 2000x500, %d steps: tiling, two-step, batch, linearity" % (name, NST), flush=True)
+    b, psv, sh = build(name)
+    psv2, sh2 = 2.0 * psv, 2.0 * sh
+    pa, pb = problem(b, psv, sh), problem(b, psv2, sh2)
+    (f1,), _ = fields([pa], tile_z=8)
+    peak = np.abs(f1).max()
+    assert np.isfinite(f1).all() and peak > 1e-3, peak      # the pulse is in the box
+    (f2,), _ = fields([pa], tile_z=32)
+    assert np.array_equal(f2.view(np.uint32), f1.view(np.uint32))
+    (f3,), s3 = fields([pa], steps_per_sweep=2)
+    assert s3 == 2 and np.array_equal(f3.view(np.uint32), f1.view(np.uint32))
+    (fa, fb), _ = fields([pa, pb])
+    assert np.array_equal(fa.view(np.uint32), f1.view(np.uint32))
+    d = fb.astype(np.float64) - 2.0 * f1.astype(np.float64)
+    assert np.sqrt((d * d).sum() / (4.0 * (f1.astype(np.float64) ** 2).sum())) <= 1e-7
