@@ -1488,55 +1488,61 @@ struct Slot {
     bool two_ok = false;              /* may take two steps per sweep */
 };
 
-/* A tile may run the single-record kernel when each of its element rows
-   (including the halo row below and the halo column to the left) holds one
-   material record over the tile's element columns. */
-bool tile_single_record(const Slot &o, int n, int x0, int z0, int rows)
+/* Tiling of one mesh.  A tile (slot, x0, z0, rows | mixed << 16) may run
+   the single-record path when every element row it sweeps -- its own rows
+   and the halo rows (one below; two below and one above for a two-step
+   tile) -- holds one material record over the wave's element columns
+   (one halo column to the left, two for a two-step tile).  Each column of
+   tiles is cut along z so that rows that are not single-record (a layer
+   cut inside an element row) get short per-node-material tiles of their
+   own and the rest stay single-record tiles of up to tz rows. */
+struct TileGeom {
+    int xc;         /* owned element columns per tile */
+    int xhalo;      /* halo element columns to the left */
+    int ncols;      /* element columns a wave holds */
+    int lo, hi;     /* halo element rows below and above */
+};
+
+static TileGeom tile_geom(int n, bool two)
 {
-    const int nex = o.meta.nex, nez = o.meta.nez;
-    const int ex0 = x0 > 0 ? x0 - 1 : 0;
-    const int ex1 = std::min(nex, x0 - 1 + 64 / (n + 1));
-    const int ez0 = z0 > 0 ? z0 - 1 : 0;
-    const int ez1 = std::min(nez, z0 + rows);
-    for (int ez = ez0; ez < ez1; ++ez) {
-        const uint16_t r = o.emat[(size_t)ez * nex + ex0];
-        if (r == 0xFFFF) return false;
-        for (int ex = ex0 + 1; ex < ex1; ++ex)
-            if (o.emat[(size_t)ez * nex + ex] != r) return false;
-    }
-    return true;
+    if (two) return TileGeom{XC2, 2, 64 / 5, 2, 1};
+    return TileGeom{xchunk_of(n), 1, 64 / (n + 1), 1, 0};
 }
 
-/* the same for a two-step tile: element columns x0-2..x0+9 and rows
-   z0-2..z0+rows */
-bool tile2_single_record(const Slot &o, int x0, int z0, int rows)
+/* tiles of one mesh (owner slot o) for each x0, in z order */
+void build_tiles(const Slot &o, int slot, int tz, bool two, std::vector<int4> &t)
 {
     const int nex = o.meta.nex, nez = o.meta.nez;
-    const int ex0 = x0 >= 2 ? x0 - 2 : 0;
-    const int ex1 = std::min(nex, x0 - 2 + 64 / 5);
-    const int ez0 = z0 >= 2 ? z0 - 2 : 0;
-    const int ez1 = std::min(nez, z0 + rows + 1);
-    for (int ez = ez0; ez < ez1; ++ez) {
-        const uint16_t r = o.emat[(size_t)ez * nex + ex0];
-        if (r == 0xFFFF) return false;
-        for (int ex = ex0 + 1; ex < ex1; ++ex)
-            if (o.emat[(size_t)ez * nex + ex] != r) return false;
-    }
-    return true;
-}
-
-/* tiles of one slot's mesh as (slot, x0, z0, rows | mixed << 16), the mixed
-   (per-node material) tiles first since they take longest */
-void build_tiles(const Slot &o, int slot, int tz, std::vector<int4> &t)
-{
-    const int n = o.meta.hpn;
-    for (int mixed = 1; mixed >= 0; --mixed)
-        for (int z0 = 0; z0 < o.meta.nez; z0 += tz)
-            for (int x0 = 0; x0 < o.meta.nex; x0 += xchunk_of(n)) {
-                const int rows = std::min(tz, o.meta.nez - z0);
-                if (tile_single_record(o, n, x0, z0, rows) == (mixed != 0)) continue;
-                t.push_back(make_int4(slot, x0, z0, rows | (mixed << 16)));
+    const TileGeom g = tile_geom(o.meta.hpn, two);
+    std::vector<char> single(nez);
+    for (int x0 = 0; x0 < nex; x0 += g.xc) {
+        const int ex0 = std::max(x0 - g.xhalo, 0), ex1 = std::min(nex, x0 - g.xhalo + g.ncols);
+        for (int ez = 0; ez < nez; ++ez) {
+            const uint16_t r = o.emat[(size_t)ez * nex + ex0];
+            bool ok = r != 0xFFFF;
+            for (int ex = ex0 + 1; ok && ex < ex1; ++ex) ok = o.emat[(size_t)ez * nex + ex] == r;
+            single[ez] = ok;
+        }
+        /* rows [z - lo, z + rows + hi) all single-record */
+        auto tile_single = [&](int z, int rows) {
+            for (int ez = std::max(z - g.lo, 0); ez < std::min(z + rows + g.hi, nez); ++ez)
+                if (!single[ez]) return false;
+            return true;
+        };
+        for (int z = 0; z < nez;) {
+            int r = 0;
+            while (r < tz && z + r < nez && tile_single(z, r + 1)) ++r;
+            if (r > 0) {
+                t.push_back(make_int4(slot, x0, z, r));
+                z += r;
+                continue;
             }
+            r = 1;
+            while (r < tz && z + r < nez && !tile_single(z + r, 1)) ++r;
+            t.push_back(make_int4(slot, x0, z, r | (1 << 16)));
+            z += r;
+        }
+    }
 }
 
 template <class T>
@@ -1631,33 +1637,37 @@ static semsks_status upload_tables(semsks_engine *e)
         e->two = two && any_used;
         e->tuned = false;
     }
-    /* tiles, grouped by hpn, mixed (per-node material) tiles first, slots
-       interleaved per spatial tile so that concurrent waves touch the same
-       region of a shared mesh */
+    /* tiles, grouped by hpn, mixed (per-node material) tiles first since
+       they take longest, then by z, x and slot, so that concurrent waves
+       touch the same region of a shared mesh */
     std::vector<int4> t;
     int cnt[SEMSKS_MAXNP] = {0};
     e->hpn_mask = 0;
     const int tzg = e->opts.tile_z > 0 ? e->opts.tile_z : e->tz_cur;
     for (int n = 3; n <= 6; ++n) {
         e->tile_off[n] = (int)t.size();
-        int maxx = 0, maxz = 0;
-        for (auto &s : e->slots)
-            if (s.used && s.meta.hpn == n) { maxx = std::max(maxx, s.meta.nex); maxz = std::max(maxz, s.meta.nez); }
-        const int xc = e->two ? XC2 : xchunk_of(n);
-        for (int mixed = 1; mixed >= 0; --mixed)
-            for (int z0 = 0; z0 < maxz; z0 += tzg)
-                for (int x0 = 0; x0 < maxx; x0 += xc)
-                    for (size_t k = 0; k < e->slots.size(); ++k) {
-                        const Slot &s = e->slots[k];
-                        if (!s.used || s.meta.hpn != n) continue;
-                        if (z0 >= s.meta.nez || x0 >= s.meta.nex) continue;
-                        const Slot &o = s.share >= 0 ? e->slots[s.share] : s;
-                        const int rows = std::min(tzg, s.meta.nez - z0);
-                        const bool single = e->two ? tile2_single_record(o, x0, z0, rows)
-                                                   : tile_single_record(o, n, x0, z0, rows);
-                        if (single == (mixed != 0)) continue;
-                        t.push_back(make_int4((int)k, x0, z0, rows | (mixed << 16)));
-                    }
+        std::vector<int4> tn, own;
+        std::map<int, std::vector<int4>> by_owner;   /* owner slot -> its tiling (slot field = owner) */
+        for (size_t k = 0; k < e->slots.size(); ++k) {
+            const Slot &s = e->slots[k];
+            if (!s.used || s.meta.hpn != n) continue;
+            const int ow = s.share >= 0 ? s.share : (int)k;
+            auto it = by_owner.find(ow);
+            if (it == by_owner.end()) {
+                own.clear();
+                build_tiles(e->slots[ow], ow, tzg, e->two, own);
+                it = by_owner.emplace(ow, own).first;
+            }
+            for (int4 q : it->second) tn.push_back(make_int4((int)k, q.y, q.z, q.w));
+        }
+        std::stable_sort(tn.begin(), tn.end(), [](const int4 &a, const int4 &b) {
+            const int ma = a.w >> 16, mb = b.w >> 16;
+            if (ma != mb) return ma > mb;
+            if (a.z != b.z) return a.z < b.z;
+            if (a.y != b.y) return a.y < b.y;
+            return a.x < b.x;
+        });
+        t.insert(t.end(), tn.begin(), tn.end());
         cnt[n] = (int)t.size() - e->tile_off[n];
         if (cnt[n]) e->hpn_mask |= 1u << n;
     }
@@ -2321,7 +2331,7 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     int4 *tl = nullptr;
     int *nt = nullptr;
     std::vector<int4> t;
-    build_tiles(s.share >= 0 ? e->slots[s.share] : s, 0, tile_rows(e, s.meta.nez), t);
+    build_tiles(s.share >= 0 ? e->slots[s.share] : s, 0, tile_rows(e, s.meta.nez), false, t);
     const int cnt = (int)t.size();
     HIPCHK(hipMalloc(&dd, sizeof d));
     HIPCHK(hipMalloc(&tl, t.size() * sizeof(int4)));
