@@ -1509,8 +1509,10 @@ static TileGeom tile_geom(int n, bool two)
     return TileGeom{xchunk_of(n), 1, 64 / (n + 1), 1, 0};
 }
 
-/* tiles of one mesh (owner slot o) for each x0, in z order */
-void build_tiles(const Slot &o, int slot, int tz, bool two, std::vector<int4> &t)
+/* walks the tiling of one mesh (owner slot o): emit(x0, z0, rows, mixed)
+   for each x0 column in z order */
+template <class F>
+void walk_tiles(const Slot &o, int tz, bool two, F emit)
 {
     const int nex = o.meta.nex, nez = o.meta.nez;
     const TileGeom g = tile_geom(o.meta.hpn, two);
@@ -1533,16 +1535,23 @@ void build_tiles(const Slot &o, int slot, int tz, bool two, std::vector<int4> &t
             int r = 0;
             while (r < tz && z + r < nez && tile_single(z, r + 1)) ++r;
             if (r > 0) {
-                t.push_back(make_int4(slot, x0, z, r));
+                emit(x0, z, r, 0);
                 z += r;
                 continue;
             }
             r = 1;
             while (r < tz && z + r < nez && !tile_single(z + r, 1)) ++r;
-            t.push_back(make_int4(slot, x0, z, r | (1 << 16)));
+            emit(x0, z, r, 1);
             z += r;
         }
     }
+}
+
+void build_tiles(const Slot &o, int slot, int tz, bool two, std::vector<int4> &t)
+{
+    walk_tiles(o, tz, two, [&](int x0, int z0, int rows, int mixed) {
+        t.push_back(make_int4(slot, x0, z0, rows | (mixed << 16)));
+    });
 }
 
 template <class T>
@@ -1717,6 +1726,19 @@ static int persistent_blocks(const semsks_engine *e, int n)
     return e->n_cu * per_cu;
 }
 
+/* the persistent grid of the two-step kernel */
+static int step2_blocks(const semsks_engine *e)
+{
+    static int per_cu = 0;
+    if (e->opts.waves_per_cu > 0) return std::max(1, e->n_cu * e->opts.waves_per_cu / 4);
+    if (per_cu < 1 &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&step2_kernel<0>), 256, 0) !=
+             hipSuccess ||
+         per_cu < 1))
+        per_cu = 2;
+    return e->n_cu * per_cu;
+}
+
 /* one launch of the element-force kernel for the tiles of polynomial degree n */
 template <bool FORCE_ONLY>
 static void launch_tiles(const semsks_engine *e, int n, const InstDesc *desc, const int *levels, const int4 *tl,
@@ -1741,16 +1763,9 @@ static semsks_status launch_step(semsks_engine *e, int smax, hipEvent_t ta, hipE
     hipLaunchKernelGGL(inject_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
     if (ta) HIPCHK(hipEventRecord(ta, e->stream));
     if (e->two) {
-        if (e->hpn_mask & (1u << 4)) {
-            static int per_cu = 0;
-            if (per_cu < 1 &&
-                (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&step2_kernel<0>),
-                                                              256, 0) != hipSuccess || per_cu < 1))
-                per_cu = 2;
-            const int nb = e->opts.waves_per_cu > 0 ? std::max(1, e->n_cu * e->opts.waves_per_cu / 4) : e->n_cu * per_cu;
-            hipLaunchKernelGGL(step2_kernel<0>, dim3(nb), dim3(256), 0, e->stream, e->d_desc, e->d_levels,
+        if (e->hpn_mask & (1u << 4))
+            hipLaunchKernelGGL(step2_kernel<0>, dim3(step2_blocks(e)), dim3(256), 0, e->stream, e->d_desc, e->d_levels,
                                e->d_tiles + e->tile_off[4], e->d_ntiles + 4, smax);
-        }
     } else {
         for (int n = 3; n <= 6; ++n)
             if (e->hpn_mask & (1u << n))
@@ -2148,15 +2163,17 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     e->timed_steps = 0;
     int done = 0;
     /* autotuner: with tile_z unset, the first call that has the steps to
-       spare runs three real launches at each candidate tile height and
-       keeps the height whose faster of the last two was fastest (the tiling
-       does not change the arithmetic, so these steps are ordinary steps of
-       the run) */
-    static const int cand1[] = {4, 8, 16, 32}, cand2[] = {8, 16, 32, 64};
-    const int *cand = e->two ? cand2 : cand1;
+       spare runs three real launches at each of four candidate tile
+       heights and keeps the height whose faster of the last two was
+       fastest (the tiling does not change the arithmetic, so these steps
+       are ordinary steps of the run).  Measured on the 2000x500 mesh: one
+       instance is fastest at 4 rows (many tiles per wave balance the
+       persistent grid), a 64-instance batch is insensitive to the height */
     const int ncand = 4;
     const int S = e->two ? 2 : 1;     /* steps per launch */
     if (e->opts.tile_z <= 0 && !e->tuned && nsteps >= 4 * ncand * S && sync != 2) {
+        static const int cand1[] = {4, 8, 16, 32}, cand2[] = {8, 16, 32, 64};
+        const int *cand = e->two ? cand2 : cand1;
         while ((int)e->tev.size() < 6) {
             hipEvent_t v;
             HIPCHK(hipEventCreate(&v));
