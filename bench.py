@@ -21,6 +21,9 @@ serving     = continuous batching (semsks_serve) of Poisson-arriving requests:
               300 s of full-size ones)
 traffic     = measured HBM bytes per launch, from the rocprofv3 summary in
               profiles/ for this workload
+precision   = the 200x100-element case's 100 seismograms, FP32 on the GPU
+              against the FP64 oracle's (a committed fixture): rel-L2 and
+              peak misfit, worst and median, per component
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3|c2|c1b|c5]
 """
@@ -125,6 +128,41 @@ def run_serving(rate, duration, nex_range, nez_range, nst_range, budget_gb=200.0
             "hbm_budget_gb": budget_gb,
             "mesh_elements": "%dx%d..%dx%d" % (nex_range[0], nez_range[0], nex_range[1], nez_range[1]),
             "steps": "%d..%d" % nst_range}
+
+
+def precision_c1():
+    """FP32 seismograms of the 200x100-element case (4000 steps, 50
+    receivers) against the FP64 oracle's, which tests/golden/make_c1_fp64.py
+    wrote to tests/golden/c1_fp64_traces.npz: rel-L2 and peak misfit of every
+    seismogram, worst and median over the receivers, per component."""
+    import tempfile
+    import semsks
+    from semsks import configs
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "c1_fp64_traces.npz"))
+    case = configs.c1_case()
+    old = os.getcwd()
+    with tempfile.TemporaryDirectory() as d:
+        case.write(d)
+        os.chdir(d)
+        try:
+            m = semsks.Model(["par=run.par"])
+            with semsks.Engine() as e:
+                s = e.admit(m.problem)
+                e.advance(m.problem.nst - 1)
+                tr = e.traces(s)
+        finally:
+            os.chdir(old)
+    out = {"case": "200x100 elements, 3 flat isotropic layers, plane SV p=0.06, 4000 steps, 50 receivers, P-SV",
+           "reference": "FP64 oracle (tests/golden/c1_fp64_traces.npz)", "seismograms": int(tr.shape[2]) * 2}
+    for name, k in (("u", 0), ("w", 1)):
+        a = tr[:, k].astype(np.float64)
+        b = ref["u" if k == 0 else "mw"].astype(np.float64)
+        l2 = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+        pk = np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)
+        out[name] = {"rel_l2_max": float(l2.max()), "rel_l2_median": float(np.median(l2)),
+                     "peak_misfit_max": float(pk.max()), "peak_misfit_median": float(np.median(pk))}
+    out["v"] = "zero in both (P-SV solver)" if np.abs(tr[:, 2]).max() == 0 else "nonzero"
+    return out
 
 
 def kernel_time(eng, nsteps):
@@ -249,6 +287,11 @@ def main():
             log("  %d requests, p50 %.3f s, p99 %.3f s, %.3e pt-steps/s" % (sv["completed"], sv["p50_latency_s"],
                                                                          sv["p99_latency_s"], sv["throughput"]))
             cfg["serving"] = sv
+        if args.only is None:
+            log("precision: 200x100 case, 4000 steps, FP32 GPU vs FP64 oracle")
+            cfg["precision"] = precision_c1()
+            log("  u rel-L2 max %.2e, w rel-L2 max %.2e" % (cfg["precision"]["u"]["rel_l2_max"],
+                                                         cfg["precision"]["w"]["rel_l2_max"]))
         if r1p is not None:
             cfg.update({"c1_mesh_plain_value": r1p["throughput"], "c1_mesh_batched_value": r1b["throughput"],
                         "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
