@@ -882,20 +882,24 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     const int ez0 = z0 > 0 ? z0 - 1 : 0;
     const int ez1 = z0 + rows;
 
-    auto fetch = [&](int ez, RowFetch<N> &r) {
+    /* share: lane b = 0 skips reading node row 0 of element row ez, the
+       row that lane b = N of the row below already holds (take_top then
+       moves it across lanes) */
+    auto fetch = [&](int ez, RowFetch<N> &r, bool share) {
         const int iz = ez * N + b;
         const long rowoff = (long)iz * ld + (long)exl * N;
+        const bool mem = !(share && b == 0);
         /* N nodes of 16 bytes: x, y, z, packed compensation */
         const SEMSKS_GMEM float4 *v4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(U + 4 * rowoff);
 #pragma unroll
         for (int a = 0; a < N; ++a) {
             float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (valid) t = v4[a];
+            if (valid && mem) t = v4[a];
             r.u[a][0] = t.x; r.u[a][1] = t.y; r.u[a][2] = t.z; r.c[a] = __float_as_uint(t.w);
         }
         {
             float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (direct) t = v4[N];
+            if (direct && mem) t = v4[N];
             r.un[0] = t.x; r.un[1] = t.y; r.un[2] = t.z; r.cn = __float_as_uint(t.w);
         }
         r.rid = 0;
@@ -920,6 +924,25 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
             }
         }
     };
+    auto take_top = [&](const RowFetch<N> &below, RowFetch<N> &r) {
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float v = __shfl(below.u[a][k], lane + N, 64);
+                if (b == 0) r.u[a][k] = v;
+            }
+            const uint32_t w = (uint32_t)__shfl((int)below.c[a], lane + N, 64);
+            if (b == 0) r.c[a] = w;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float v = __shfl(below.un[k], lane + N, 64);
+            if (b == 0) r.un[k] = v;
+        }
+        const uint32_t w = (uint32_t)__shfl((int)below.cn, lane + N, 64);
+        if (b == 0) r.cn = w;
+    };
 
     /* carried raw top-row force of the element below, one block per element */
     if (act && b == N) {
@@ -932,10 +955,15 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
        per-node material records in registers, fetches at the top of a row */
     constexpr bool AHEAD = !MIXED;
     RowFetch<N> cur;
-    fetch(ez0, cur);
+    fetch(ez0, cur, false);
     for (int ez = ez0; ez < ez1; ++ez) {
         if constexpr (!AHEAD) {
-            if (ez > ez0) fetch(ez, cur);
+            if (ez > ez0) {
+                RowFetch<N> nx;
+                fetch(ez, nx, true);
+                take_top(cur, nx);
+                cur = nx;
+            }
         }
         const int iz = ez * N + b;
         const bool top = ez == nez - 1;
@@ -1005,7 +1033,10 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
         RowFetch<N> nxt;
         if constexpr (AHEAD) {
             __builtin_amdgcn_sched_barrier(0);
-            if (ez + 1 < ez1) fetch(ez + 1, nxt);
+            if (ez + 1 < ez1) {
+                fetch(ez + 1, nxt, true);
+                take_top(cur, nxt);
+            }
             __builtin_amdgcn_sched_barrier(0);
         }
         if (own) {
