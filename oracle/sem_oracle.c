@@ -738,7 +738,11 @@ static void absorbs(const orc_ctx *c, orc_vec *p1, const orc_vec *p, const orc_v
     for (i = 0; i < nz; i++) absorb_node(c, i * nx, p1, p, p0, G, dt2);
     for (i = 0; i < nx; i++) absorb_node(c, i, p1, p, p0, G, dt2);
     for (i = 0; i < nz; i++) absorb_node(c, i * nx + nx - 1, p1, p, p0, G, dt2);
-    for (i = 1; i < 3 * c->hpn + 1; i++) absorb_node(c, (nz - 1) * nx + i, p1, p, p0, G, dt2);
+    /* the top-row strip ix = 1..3*hpn; the reference's loop does not stop at
+       the row end, which only matters (as an overrun) for nx <= 3*hpn+1,
+       i.e. meshes of fewer than 3 element columns: clipped here as in the
+       product */
+    for (i = 1; i < 3 * c->hpn + 1 && i < nx; i++) absorb_node(c, (nz - 1) * nx + i, p1, p, p0, G, dt2);
 }
 
 /* atten_wave2 (sem2d_psvsh.c:1780-1823) with atten_taper (:1725-1738) */
