@@ -429,8 +429,10 @@ __device__ __forceinline__ bool is_inject_node(int ix, int iz, int nbl)
     return (iz == nbl && ix >= nbl) || (ix == nbl && iz > nbl);
 }
 
-__global__ void inject_kernel(const InstDesc *insts, const int *levels)
+__global__ void inject_kernel(const InstDesc *insts, const int *levels, int *qctr)
 {
+    /* the step kernel that follows starts its tile queues at zero */
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < SEMSKS_QCOUNTERS) qctr[threadIdx.x * SEMSKS_QSTRIDE] = 0;
     const InstDesc &D = insts[blockIdx.y];
     if (!D.active) return;
     const int it = levels[blockIdx.y];
@@ -1086,41 +1088,90 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     }
 }
 
-/* persistent grid: each 64-lane wave takes tiles (slot, x0, z0, rows | mixed
-   << 16) from the work list until it is exhausted; a mixed tile takes the
-   per-node material path */
+/* Blocks of a launch are dealt round-robin to the 8 XCDs of an MI355X
+   (block b runs on XCD b % 8). */
+constexpr int NXCD = SEMSKS_NQ;
+
+/* one tile of the step kernel: (slot, x0, z0, rows | mixed << 16) */
+template <int N, bool FORCE_ONLY>
+__device__ __forceinline__ void step_tile(const InstDesc *__restrict__ insts, const int *__restrict__ levels, int4 tile,
+                                          int lane, float *lds)
+{
+    tile.x = __builtin_amdgcn_readfirstlane(tile.x);
+    tile.y = __builtin_amdgcn_readfirstlane(tile.y);
+    tile.z = __builtin_amdgcn_readfirstlane(tile.z);
+    tile.w = __builtin_amdgcn_readfirstlane(tile.w);
+    const InstDesc &D = insts[tile.x];
+    if (!D.active || D.hpn != N) return;
+    int p = 0;
+    if constexpr (!FORCE_ONLY) {
+        const int it = levels[tile.x];
+        if (it >= D.nst - 1) return;
+        p = it % D.nlev;
+    }
+    const int rows = tile.w & 0xFFFF;
+    if (D.ortho) {
+        if (tile.w >> 16) process_tile<N, FORCE_ONLY, true, true>(D, p, tile.y, tile.z, rows, lane, lds);
+        else process_tile<N, FORCE_ONLY, false, true>(D, p, tile.y, tile.z, rows, lane, lds);
+    } else {
+        if (tile.w >> 16) process_tile<N, FORCE_ONLY, true, false>(D, p, tile.y, tile.z, rows, lane, lds);
+        else process_tile<N, FORCE_ONLY, false, false>(D, p, tile.y, tile.z, rows, lane, lds);
+    }
+}
+
+/* Persistent grid over the tile list (slot, x0, z0, rows | mixed << 16); a
+   mixed tile takes the per-node material path.  The list is cut into NXCD
+   contiguous runs, one per XCD, so that consecutive tiles -- the
+   neighbours of one instance, which read the same 128-byte lines along
+   their shared edge -- run at the same time under the same L2.
+   Time steps with more tiles than waves (qctr != nullptr): each run is a
+   queue with its counter in qctr[q * SEMSKS_QSTRIDE] (zero at launch:
+   inject_kernel clears them).  A wave takes the next tile of its own XCD's
+   queue, which keeps neighbours together however far the waves drift
+   apart; a wave whose queue is empty takes from the next queues in turn
+   and leaves once all NXCD are exhausted.  Otherwise the waves of an XCD
+   deal its run statically (local wave w takes w, w + W, ...). */
 template <int N, bool FORCE_ONLY>
 __global__ __launch_bounds__(256, N <= 4 ? 3 : 2) void step_kernel(const InstDesc *__restrict__ insts, const int *__restrict__ levels,
-                                                      const int4 *__restrict__ tiles, const int *__restrict__ ntiles)
+                                                      const int4 *__restrict__ tiles, const int *__restrict__ ntiles,
+                                                      int *__restrict__ qctr)
 {
     __shared__ float lds_blocks[4][RowLayout<N>::WAVE_FLOATS];
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    float *lds = lds_blocks[threadIdx.x >> 6];
+    const int wpb = blockDim.x >> 6;
     const int nt = *ntiles;
-    for (int t = wave; t < nt; t += nwaves) {
-        int4 tile = tiles[t];
-        tile.x = __builtin_amdgcn_readfirstlane(tile.x);
-        tile.y = __builtin_amdgcn_readfirstlane(tile.y);
-        tile.z = __builtin_amdgcn_readfirstlane(tile.z);
-        tile.w = __builtin_amdgcn_readfirstlane(tile.w);
-        const InstDesc &D = insts[tile.x];
-        if (!D.active || D.hpn != N) continue;
-        int p = 0;
-        if constexpr (!FORCE_ONLY) {
-            const int it = levels[tile.x];
-            if (it >= D.nst - 1) continue;
-            p = it % D.nlev;
-        }
-        float *lds = lds_blocks[threadIdx.x >> 6];
-        const int rows = tile.w & 0xFFFF;
-        if (D.ortho) {
-            if (tile.w >> 16) process_tile<N, FORCE_ONLY, true, true>(D, p, tile.y, tile.z, rows, lane, lds);
-            else process_tile<N, FORCE_ONLY, false, true>(D, p, tile.y, tile.z, rows, lane, lds);
+    const bool split = gridDim.x % NXCD == 0;
+    const int nq = split ? NXCD : 1;
+    const int q0 = split ? blockIdx.x % NXCD : 0;
+    const int W = (split ? gridDim.x / NXCD : gridDim.x) * wpb;        /* waves per run */
+    const bool queued = qctr != nullptr && nt > W * nq;
+    int qi = 0;
+    /* run q is [nt * q / nq, nt * (q + 1) / nq) (nt < 2^28: one sweep holds at most a few million tiles) */
+    int t = nt * q0 / nq + (split ? blockIdx.x / NXCD : blockIdx.x) * wpb + (threadIdx.x >> 6) - W;
+    t = __builtin_amdgcn_readfirstlane(t);
+    const int shi = nt * (q0 + 1) / nq;
+    for (;;) {
+        if (queued) {
+            t = -1;
+            while (qi < nq) {
+                const int q = (q0 + qi) % nq;
+                const int lo = nt * q / nq, hi = nt * (q + 1) / nq;
+                int k = 0;
+                if (lane == 0) k = atomicAdd(qctr + q * SEMSKS_QSTRIDE, 1);
+                k = __builtin_amdgcn_readfirstlane(k);
+                if (lo + k < hi) {
+                    t = lo + k;
+                    break;
+                }
+                ++qi;
+            }
+            if (t < 0) break;
         } else {
-            if (tile.w >> 16) process_tile<N, FORCE_ONLY, true, false>(D, p, tile.y, tile.z, rows, lane, lds);
-            else process_tile<N, FORCE_ONLY, false, false>(D, p, tile.y, tile.z, rows, lane, lds);
+            t += W;
+            if (t >= shi) break;
         }
+        step_tile<N, FORCE_ONLY>(insts, levels, tiles[t], lane, lds);
     }
 }
 
@@ -1607,6 +1658,7 @@ struct semsks_engine {
     int *d_levels = nullptr;
     int4 *d_tiles = nullptr;
     int *d_ntiles = nullptr;          /* [7]: per hpn 3..6 (index hpn) */
+    int *d_qctr = nullptr;            /* tile-queue counters, SEMSKS_QCOUNTERS lines */
     std::vector<int> h_levels;
     int tile_cap = 0;
     int tile_off[SEMSKS_MAXNP] = {0}; /* offset of each hpn's tiles */
@@ -1678,8 +1730,10 @@ static semsks_status upload_tables(semsks_engine *e)
         e->tuned = false;
     }
     /* tiles, grouped by hpn, mixed (per-node material) tiles first since
-       they take longest, then by z, x and slot, so that concurrent waves
-       touch the same region of a shared mesh */
+       they take longest, then by z, slot and x: the waves of one XCD,
+       which take consecutive tiles (see step_kernel), sweep neighbouring
+       tiles of one instance side by side and share the lines along their
+       edges in L2 */
     std::vector<int4> t;
     int cnt[SEMSKS_MAXNP] = {0};
     e->hpn_mask = 0;
@@ -1704,13 +1758,15 @@ static semsks_status upload_tables(semsks_engine *e)
             const int ma = a.w >> 16, mb = b.w >> 16;
             if (ma != mb) return ma > mb;
             if (a.z != b.z) return a.z < b.z;
-            if (a.y != b.y) return a.y < b.y;
-            return a.x < b.x;
+            if (a.x != b.x) return a.x < b.x;
+            return a.y < b.y;
         });
         t.insert(t.end(), tn.begin(), tn.end());
         cnt[n] = (int)t.size() - e->tile_off[n];
         if (cnt[n]) e->hpn_mask |= 1u << n;
     }
+    if (t.size() >= (1u << 28))       /* step_kernel's run bounds nt * q / NXCD are 32-bit */
+        return semsks_set_error(SEMSKS_EINVAL, "tile list of %zu tiles is too long", t.size());
     if ((int)t.size() > e->tile_cap) {
         drop_graph(e);                /* a captured step holds the old list */
         HIPCHK(hipStreamSynchronize(e->stream));
@@ -1777,11 +1833,14 @@ static void launch_tiles(const semsks_engine *e, int n, const InstDesc *desc, co
 {
     const int nb = persistent_blocks<FORCE_ONLY>(e, n);
     hipStream_t st = e->stream;
+    /* time steps take tiles from the queues (cleared by inject_kernel just
+       before); the force-only launch deals them statically */
+    int *q = FORCE_ONLY ? nullptr : e->d_qctr + n * SEMSKS_NQ * SEMSKS_QSTRIDE;
     switch (n) {
-    case 3: hipLaunchKernelGGL((step_kernel<3, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt); break;
-    case 4: hipLaunchKernelGGL((step_kernel<4, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt); break;
-    case 5: hipLaunchKernelGGL((step_kernel<5, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt); break;
-    case 6: hipLaunchKernelGGL((step_kernel<6, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt); break;
+    case 3: hipLaunchKernelGGL((step_kernel<3, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt, q); break;
+    case 4: hipLaunchKernelGGL((step_kernel<4, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt, q); break;
+    case 5: hipLaunchKernelGGL((step_kernel<5, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt, q); break;
+    case 6: hipLaunchKernelGGL((step_kernel<6, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt, q); break;
     }
 }
 
@@ -1791,7 +1850,7 @@ static semsks_status launch_step(semsks_engine *e, int smax, hipEvent_t ta, hipE
 {
     const int nslots = (int)e->slots.size();
     if (!e->two) smax = 1;
-    hipLaunchKernelGGL(inject_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
+    hipLaunchKernelGGL(inject_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels, e->d_qctr);
     if (ta) HIPCHK(hipEventRecord(ta, e->stream));
     if (e->two) {
         if (e->hpn_mask & (1u << 4))
@@ -1862,6 +1921,8 @@ semsks_status semsks_engine_create(const semsks_engine_opts *opts, semsks_engine
     HIPCHK(hipMalloc(&e->d_desc, e->slots.size() * sizeof(InstDesc)));
     HIPCHK(hipMalloc(&e->d_levels, e->slots.size() * sizeof(int)));
     HIPCHK(hipMalloc(&e->d_ntiles, SEMSKS_MAXNP * sizeof(int)));
+    HIPCHK(hipMalloc(&e->d_qctr, SEMSKS_QCOUNTERS * SEMSKS_QSTRIDE * sizeof(int)));
+    HIPCHK(hipMemset(e->d_qctr, 0, SEMSKS_QCOUNTERS * SEMSKS_QSTRIDE * sizeof(int)));
     std::vector<int> z(e->slots.size(), 0);
     HIPCHK(hipMemcpy(e->d_levels, z.data(), z.size() * sizeof(int), hipMemcpyHostToDevice));
     semsks_status st = upload_tables(e);
@@ -1887,7 +1948,7 @@ void semsks_engine_destroy(semsks_engine *e)
     for (size_t k = 0; k < e->slots.size(); ++k) if (e->slots[k].used) free_slot(e, (int)k);
     if (e->pend_psv) hipFree(e->pend_psv);
     if (e->pend_sh) hipFree(e->pend_sh);
-    hipFree(e->d_desc); hipFree(e->d_levels); hipFree(e->d_tiles); hipFree(e->d_ntiles);
+    hipFree(e->d_desc); hipFree(e->d_levels); hipFree(e->d_tiles); hipFree(e->d_ntiles); hipFree(e->d_qctr);
     for (auto &v : e->ev) if (v) hipEventDestroy(v);
     for (auto &v : e->tev) hipEventDestroy(v);
     if (e->stream) hipStreamDestroy(e->stream);

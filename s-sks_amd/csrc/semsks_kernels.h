@@ -26,6 +26,12 @@
 
 #define SEMSKS_MAXNP 7
 
+/* tile-queue counters of the step kernel: SEMSKS_NQ per polynomial degree
+   (index hpn * SEMSKS_NQ + queue), each on its own 128-byte line */
+#define SEMSKS_NQ 8
+#define SEMSKS_QSTRIDE 32
+#define SEMSKS_QCOUNTERS (SEMSKS_MAXNP * SEMSKS_NQ)
+
 struct InstDesc {
     int nex, nez, nx, nz, ld;
     int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active, nlev;
