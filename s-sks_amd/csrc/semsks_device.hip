@@ -333,7 +333,7 @@ __global__ void init_state_kernel(InstDesc D)
             q = (long)iz * ld + nbl;
             u = g[2 * nx + iz]; w = -g[2 * nx + nz + iz]; if (sh) v = gv[nx + iz];
         }
-        D.u[0][4 * q] = u; D.u[0][4 * q + 1] = v; D.u[0][4 * q + 2] = w;
+        D.u[0][3 * q] = u; D.u[0][3 * q + 1] = v; D.u[0][3 * q + 2] = w;
     }
 }
 
@@ -346,45 +346,66 @@ __device__ __forceinline__ void kahan_add(float &u, float &c, float x)
 }
 
 /* The Kahan compensation c of u is the rounding error of the last addition,
-   |c| <= ulp(u)/2.  It is stored per component as a signed byte in units of
-   ulp(u)/256 (2^(e-32) with u = m 2^e, 0.5 <= |m| < 1), three components
-   per 32-bit word; the residual it leaves is below ulp(u)/512 per step. */
-__device__ __forceinline__ float c_unpack(uint32_t w, int k, float u)
+   |c| <= ulp(u)/2 (true value u - c).  It is stored per component as a
+   signed 5-bit field in units of ulp(u)/32 (2^(e-29) with u = m 2^e,
+   0.5 <= |m| < 1), clamped to +-15; the residual it leaves is below
+   ulp(u)/64 per step.  The 32-bit word of level n holds the fields of level
+   n (bits 0-14, component k at 5k) and those of level n-1 (bits 15-29), so
+   a step reads the word of level n only: the previous increment
+   U_n - U_{n-1} needs both, the Kahan sum the first. */
+constexpr uint32_t C_OWN = 0x7fffu;          /* fields of the word's own level */
+
+__device__ __forceinline__ float c_field(uint32_t w, int f, float u)
 {
-    const int q = (int)(int8_t)((w >> (8 * k)) & 0xffu);
-    return ldexpf((float)q, __builtin_amdgcn_frexp_expf(u) - 32);
+    const int q = (int)(w << (27 - 5 * f)) >> 27;
+    return ldexpf((float)q, __builtin_amdgcn_frexp_expf(u) - 29);
 }
 
 __device__ __forceinline__ int c_quant(float c, float u)
 {
-    const int q = (int)rintf(ldexpf(c, 32 - __builtin_amdgcn_frexp_expf(u)));
-    return q > 127 ? 127 : (q < -127 ? -127 : q);
+    const int q = (int)rintf(ldexpf(c, 29 - __builtin_amdgcn_frexp_expf(u)));
+    return q > 15 ? 15 : (q < -15 ? -15 : q);
 }
 
 __device__ __forceinline__ uint32_t c_pack3(int q0, int q1, int q2)
 {
-    const uint32_t t = __builtin_amdgcn_perm((uint32_t)q1, (uint32_t)q0, 0x0c0c0400u);   /* q0.b0, q1.b0 */
-    return __builtin_amdgcn_perm((uint32_t)q2, t, 0x0c040100u);                         /* t.b0, t.b1, q2.b0 */
+    return ((uint32_t)q0 & 31u) | (((uint32_t)q1 & 31u) << 5) | (((uint32_t)q2 & 31u) << 10);
 }
 
-/* u += d for the three components, the compensation packed in w carried
-   along (w = 0 without it) */
+/* u += d for the three components.  w is the word of u's level; ADVANCE:
+   u becomes the next level, whose word takes the new fields and keeps u's
+   old ones as its previous-level fields; otherwise (an in-place change of
+   the same level, the interface injection) the previous-level fields stay.
+   Without compensation w stays 0. */
+template <bool ADVANCE>
 __device__ __forceinline__ void kahan3(bool kahan, float (&u)[3], uint32_t &w, const float (&d)[3])
 {
     if (kahan) {
         int q[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            float c = c_unpack(w, k, u[k]);
+            float c = c_field(w, k, u[k]);
             kahan_add(u[k], c, d[k]);
             q[k] = c_quant(c, u[k]);
         }
-        w = c_pack3(q[0], q[1], q[2]);
+        const uint32_t hi = ADVANCE ? (w & C_OWN) << 15 : (w & ~C_OWN);
+        w = c_pack3(q[0], q[1], q[2]) | hi;
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) u[k] = u[k] + d[k];
         w = 0u;
     }
+}
+
+/* a node's xyz at float offset 3q of a level plane */
+__device__ __forceinline__ void xyz_load(const SEMSKS_GMEM float *P, long q, float (&v)[3])
+{
+    v[0] = P[3 * q]; v[1] = P[3 * q + 1]; v[2] = P[3 * q + 2];
+}
+
+__device__ __forceinline__ void xyz_store(SEMSKS_GMEM float *P, long q, const float (&v)[3])
+{
+    P[3 * q] = v[0]; P[3 * q + 1] = v[1]; P[3 * q + 2] = v[2];
 }
 
 /* GRT_Interface4: u += g(t) - g(t-dt) (w sign flipped); GRT_Interface3:
@@ -441,6 +462,7 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels, int *qct
     const int L = D.nlev, p = it % L, nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld;
     const bool mp = D.solver == SEMSKS_SOLVER_PSV_MP;
     float *U = D.u[p];
+    uint32_t *W = D.comp[p];
     const float *Up = D.u[(p + L - 1) % L];
     const int nb = nx - nbl, nl = nz - nbl - 1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb + nl; i += gridDim.x * blockDim.x) {
@@ -448,12 +470,12 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels, int *qct
         const int iz = i < nb ? nbl : nbl + 1 + (i - nb);
         const long q = (long)iz * ld + ix;
         float d[3];
-        inject_delta(D, it, ix, iz, mp ? Up[4 * q] : 0.f, mp ? Up[4 * q + 2] : 0.f, d);
-        float u[3] = {U[4 * q], U[4 * q + 1], U[4 * q + 2]};
-        uint32_t w = D.kahan ? __float_as_uint(U[4 * q + 3]) : 0u;
-        kahan3(D.kahan != 0, u, w, d);
-        for (int k = 0; k < 3; ++k) U[4 * q + k] = u[k];
-        U[4 * q + 3] = __uint_as_float(w);
+        inject_delta(D, it, ix, iz, mp ? Up[3 * q] : 0.f, mp ? Up[3 * q + 2] : 0.f, d);
+        float u[3] = {U[3 * q], U[3 * q + 1], U[3 * q + 2]};
+        uint32_t w = W[q];
+        kahan3<false>(D.kahan != 0, u, w, d);
+        for (int k = 0; k < 3; ++k) U[3 * q + k] = u[k];
+        W[q] = w;
     }
 }
 
@@ -489,9 +511,9 @@ __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
             f = D.taper[iz - nz + nh];
         }
         const long q = (long)iz * ld + ix;
-        float *U1 = D.u[pn] + 4 * q;
+        float *U1 = D.u[pn] + 3 * q;
         for (int k = 0; k < 3; ++k) U1[k] = f * U1[k];
-        U1[3] = 0.f;
+        D.comp[pn][q] &= ~C_OWN;          /* the previous level's fields stay */
     }
 }
 
@@ -509,9 +531,9 @@ __global__ void record_kernel(const InstDesc *insts, int *levels, int smax)
         float *tr = D.traces + (long)lev * 3 * nt;
         for (int r = threadIdx.x; r < nt; r += blockDim.x) {
             long q = D.rec[r];
-            tr[r] = D.u[p][4 * q];
-            tr[nt + r] = -D.u[p][4 * q + 2];
-            tr[2 * nt + r] = D.u[p][4 * q + 1];
+            tr[r] = D.u[p][3 * q];
+            tr[nt + r] = -D.u[p][3 * q + 2];
+            tr[2 * nt + r] = D.u[p][3 * q + 1];
         }
     }
     __syncthreads();
@@ -523,9 +545,9 @@ __global__ void record0_kernel(InstDesc D)
     const int nt = D.ntrace;
     for (int r = threadIdx.x; r < nt; r += blockDim.x) {
         long q = D.rec[r];
-        D.traces[r] = D.u[0][4 * q];
-        D.traces[nt + r] = -D.u[0][4 * q + 2];
-        D.traces[2 * nt + r] = D.u[0][4 * q + 1];
+        D.traces[r] = D.u[0][3 * q];
+        D.traces[nt + r] = -D.u[0][3 * q + 2];
+        D.traces[2 * nt + r] = D.u[0][3 * q + 1];
     }
 }
 
@@ -647,24 +669,25 @@ __device__ __forceinline__ float mdt2_of(const InstDesc &D, int mid, int ix, int
 }
 
 /* delta of the previous step from the compensated values of u(t) and
-   u(t-dt): (u - up) - (c - cp), c the compensation (true value u - c) */
-__device__ __forceinline__ float delta_of(float u, uint32_t w, float up, uint32_t wp, int k, bool kahan)
+   u(t-dt): (u - up) - (c - cp), c the compensation (true value u - c); w is
+   the word of level t, which holds both c and cp */
+__device__ __forceinline__ float delta_of(float u, uint32_t w, float up, int k, bool kahan)
 {
     const float d = u - up;
-    return kahan ? d - (c_unpack(w, k, u) - c_unpack(wp, k, up)) : d;
+    return kahan ? d - (c_field(w, k, u) - c_field(w, k + 3, up)) : d;
 }
 
-/* u(t+dt) of one node: delta from u(t) (u, packed compensation w) and
-   u(t-dt) (up, wp), then interior nodes (eidx < 0) by dt^2/M = m, edge
-   nodes by their Lysmer factors, and the Kahan-compensated sum */
+/* u(t+dt) of one node: delta from u(t) (u, word w) and u(t-dt) (up), then
+   interior nodes (eidx < 0) by dt^2/M = m, edge nodes by their Lysmer
+   factors, and the Kahan-compensated sum (word wn of level t+dt) */
 __device__ __forceinline__ void next_node(const InstDesc &D, bool kahan, int eidx, float m, const float (&f)[3],
-                                          const float (&uc)[3], uint32_t w, const float (&up)[3], uint32_t wp,
+                                          const float (&uc)[3], uint32_t w, const float (&up)[3],
                                           float (&nu)[3], uint32_t &wn)
 {
     float d[3], ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
-    d[0] = delta_of(uc[0], w, up[0], wp, 0, kahan);
-    d[1] = delta_of(uc[1], w, up[1], wp, 1, kahan);
-    d[2] = delta_of(uc[2], w, up[2], wp, 2, kahan);
+    d[0] = delta_of(uc[0], w, up[0], 0, kahan);
+    d[1] = delta_of(uc[1], w, up[1], 1, kahan);
+    d[2] = delta_of(uc[2], w, up[2], 2, kahan);
     if (eidx >= 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) { ea[k] = gp(D.edge_a)[eidx * 3 + k]; eb[k] = gp(D.edge_b)[eidx * 3 + k]; }
@@ -675,23 +698,25 @@ __device__ __forceinline__ void next_node(const InstDesc &D, bool kahan, int eid
         nu[k] = uc[k];
     }
     wn = w;
-    kahan3(kahan, nu, wn, d);
+    kahan3<true>(kahan, nu, wn, d);
 }
 
 /* one node: delta, Kahan-compensated u, edge rows by their Lysmer factors;
-   uc/w: the node's u(t) and its packed compensation; p: buffer of level t */
+   uc: the node's u(t); p: plane of level t */
 template <int N>
 __device__ __forceinline__ void update_one(const InstDesc &D, int p, long q, int ix, int iz, int mid,
-                                           const float (&f)[3], const float (&uc)[3], uint32_t w)
+                                           const float (&f)[3], const float (&uc)[3])
 {
     const int e = edge_index(ix, iz, D.nx, D.nz, D.hpn);
-    const float4 pv = *reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.u[(p + D.nlev - 1) % D.nlev]) + 4 * q);
-    const float up[3] = {pv.x, pv.y, pv.z};
+    float up[3];
+    xyz_load(gp(D.u[(p + D.nlev - 1) % D.nlev]), q, up);
+    const uint32_t w = gp(D.comp[p])[q];
     const float m = e < 0 ? mdt2_of<N>(D, mid, ix, iz) : 0.f;
     float nu[3];
     uint32_t wn;
-    next_node(D, D.kahan != 0, e, m, f, uc, w, up, __float_as_uint(pv.w), nu, wn);
-    *reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[(p + 1) % D.nlev]) + 4 * q) = make_float4(nu[0], nu[1], nu[2], __uint_as_float(wn));
+    next_node(D, D.kahan != 0, e, m, f, uc, w, up, nu, wn);
+    xyz_store(gp(D.u[(p + 1) % D.nlev]), q, nu);
+    gp(D.comp[(p + 1) % D.nlev])[q] = wn;
 }
 
 /* Lane layout of the element-force/update kernel: a 64-lane wave holds
@@ -845,13 +870,46 @@ __device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)
    current row's stores, so waiting for it never waits for those stores. */
 template <int N> struct RowFetch {
     float u[N][3];
-    uint32_t c[N];     /* packed compensation of u(t) */
     float un[3];
-    uint32_t cn;
-    float4 pv[4];      /* per owned node: u(t-dt) xyz, its packed compensation */
+    uint32_t w[N];     /* compensation words of level t: the owned nodes (every
+                          node of the row in a two-step sweep) */
+    uint32_t wn;
+    float pv[4][3];    /* per owned node: u(t-dt) */
     float m[4];
     int rid;           /* single-record tiles: the row's material record */
 };
+
+/* N consecutive nodes' xyz from a level plane (P at the first node); zero
+   when off.  For N = 4 the 48 bytes start on a 16-byte boundary (rows are
+   line-aligned and element columns 48 bytes apart). */
+template <int N>
+__device__ __forceinline__ void row_load(const SEMSKS_GMEM float *P, bool on, float (&v)[N][3])
+{
+    if constexpr (N == 4) {
+        float4 t0 = make_float4(0.f, 0.f, 0.f, 0.f), t1 = t0, t2 = t0;
+        if (on) {
+            const SEMSKS_GMEM float4 *p4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(P);
+            t0 = p4[0]; t1 = p4[1]; t2 = p4[2];
+        }
+        v[0][0] = t0.x; v[0][1] = t0.y; v[0][2] = t0.z; v[1][0] = t0.w;
+        v[1][1] = t1.x; v[1][2] = t1.y; v[2][0] = t1.z; v[2][1] = t1.w;
+        v[2][2] = t2.x; v[3][0] = t2.y; v[3][1] = t2.z; v[3][2] = t2.w;
+    } else {
+#pragma unroll
+        for (int a = 0; a < N; ++a)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[a][k] = on ? P[3 * a + k] : 0.f;
+    }
+}
+
+/* four nodes' xyz to a level plane (16-byte aligned, as row_load) */
+__device__ __forceinline__ void row_store4(SEMSKS_GMEM float *P, const float (&v)[4][3])
+{
+    SEMSKS_GMEM float4 *p4 = reinterpret_cast<SEMSKS_GMEM float4 *>(P);
+    p4[0] = make_float4(v[0][0], v[0][1], v[0][2], v[1][0]);
+    p4[1] = make_float4(v[1][1], v[1][2], v[2][0], v[2][1]);
+    p4[2] = make_float4(v[2][2], v[3][0], v[3][1], v[3][2]);
+}
 
 /* MIXED = false: every element row of the tile (over the wave's element
    columns) holds one material record, which the host has checked; the
@@ -891,32 +949,21 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
         const int iz = ez * N + b;
         const long rowoff = (long)iz * ld + (long)exl * N;
         const bool mem = !(share && b == 0);
-        /* N nodes of 16 bytes: x, y, z, packed compensation */
-        const SEMSKS_GMEM float4 *v4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(U + 4 * rowoff);
-#pragma unroll
-        for (int a = 0; a < N; ++a) {
-            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (valid && mem) t = v4[a];
-            r.u[a][0] = t.x; r.u[a][1] = t.y; r.u[a][2] = t.z; r.c[a] = __float_as_uint(t.w);
-        }
-        {
-            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (direct && mem) t = v4[N];
-            r.un[0] = t.x; r.un[1] = t.y; r.un[2] = t.z; r.cn = __float_as_uint(t.w);
-        }
+        row_load<N>(U + 3 * rowoff, valid && mem, r.u);
+        r.un[0] = r.un[1] = r.un[2] = 0.f;
+        if (direct && mem) xyz_load(U, rowoff + N, r.un);
         r.rid = 0;
         if constexpr (!MIXED) r.rid = (int)cp(D.emat)[(long)ez * nex + ex_first];
         if constexpr (N == 4 && !FORCE_ONLY) {
             const bool own = own_x && ez >= z0 && b < N;
             const bool vec = own && iz != 0 && ex != 0;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) r.pv[a] = make_float4(0.f, 0.f, 0.f, 0.f);
             r.m[0] = r.m[1] = r.m[2] = r.m[3] = 0.f;
+            uint4 w4 = make_uint4(0u, 0u, 0u, 0u);
+            if (vec) w4 = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(gp(D.comp[p]) + rowoff);
+            r.w[0] = w4.x; r.w[1] = w4.y; r.w[2] = w4.z; r.w[3] = w4.w;
+            row_load<4>(gp(D.u[pp]) + 3 * rowoff, vec, r.pv);
             if (vec) {
                 const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
-                const SEMSKS_GMEM float4 *s4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(gp(D.u[pp]) + 4 * rowoff);
-#pragma unroll
-                for (int a = 0; a < 4; ++a) r.pv[a] = s4[a];
                 /* mass class: cz = b (0 = row shared with the element below) */
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
@@ -934,16 +981,12 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
                 const float v = __shfl(below.u[a][k], lane + N, 64);
                 if (b == 0) r.u[a][k] = v;
             }
-            const uint32_t w = (uint32_t)__shfl((int)below.c[a], lane + N, 64);
-            if (b == 0) r.c[a] = w;
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const float v = __shfl(below.un[k], lane + N, 64);
             if (b == 0) r.un[k] = v;
         }
-        const uint32_t w = (uint32_t)__shfl((int)below.cn, lane + N, 64);
-        if (b == 0) r.cn = w;
     };
 
     /* carried raw top-row force of the element below, one block per element */
@@ -1053,25 +1096,25 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
 #pragma unroll
                         for (int k = 0; k < 3; ++k) gp(D.fout[k])[q0 + a] = f[a][k];
             } else if (N == 4 && vec) {
-                SEMSKS_GMEM float4 *u4 = reinterpret_cast<SEMSKS_GMEM float4 *>(gp(D.u[pn]) + 4 * q0);
+                float nv[4][3];
+                uint32_t nw[4];
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
-                    const uint32_t wc = cur.c[a], wp = __float_as_uint(cur.pv[a].w);
-                    const float up[3] = {cur.pv[a].x, cur.pv[a].y, cur.pv[a].z};
-                    float v[3], d[3];
+                    float d[3];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        d[k] = delta_of(u[a][k], wc, up[k], wp, k, kahan) - cur.m[a] * f[a][k];
-                        v[k] = u[a][k];
+                        d[k] = delta_of(u[a][k], cur.w[a], cur.pv[a][k], k, kahan) - cur.m[a] * f[a][k];
+                        nv[a][k] = u[a][k];
                     }
-                    uint32_t cw = wc;
-                    kahan3(kahan, v, cw, d);
-                    u4[a] = make_float4(v[0], v[1], v[2], __uint_as_float(cw));
+                    nw[a] = cur.w[a];
+                    kahan3<true>(kahan, nv[a], nw[a], d);
                 }
+                row_store4(gp(D.u[pn]) + 3 * q0, nv);
+                *reinterpret_cast<SEMSKS_GMEM uint4 *>(gp(D.comp[pn]) + q0) = make_uint4(nw[0], nw[1], nw[2], nw[3]);
                 if (last_x) {
                     float fa[3] = {f[N][0], f[N][1], f[N][2]};
                     float ua[3] = {u[N][0], u[N][1], u[N][2]};
-                    update_one<N>(D, p, q0 + N, ixb + N, iz, MIXED ? (int)idrow[N] : rid, fa, ua, cur.cn);
+                    update_one<N>(D, p, q0 + N, ixb + N, iz, MIXED ? (int)idrow[N] : rid, fa, ua);
                 }
             } else {
 #pragma unroll
@@ -1079,8 +1122,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
                     if (a >= a1) continue;
                     float fa[3] = {f[a][0], f[a][1], f[a][2]};
                     float ua[3] = {u[a][0], u[a][1], u[a][2]};
-                    update_one<N>(D, p, q0 + a, ixb + a, iz, MIXED ? (int)idrow[a] : rid, fa, ua,
-                                  a < N ? cur.c[a] : cur.cn);
+                    update_one<N>(D, p, q0 + a, ixb + a, iz, MIXED ? (int)idrow[a] : rid, fa, ua);
                 }
             }
         }
@@ -1221,6 +1263,9 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     const SEMSKS_GMEM float *UP = gp(D.u[(p + 3) % 4]);
     SEMSKS_GMEM float *U1 = gp(D.u[(p + 1) % 4]);
     SEMSKS_GMEM float *U2 = gp(D.u[(p + 2) % 4]);
+    const SEMSKS_GMEM uint32_t *W = gp(D.comp[p]);
+    SEMSKS_GMEM uint32_t *W1 = gp(D.comp[(p + 1) % 4]);
+    SEMSKS_GMEM uint32_t *W2 = gp(D.comp[(p + 2) % 4]);
 
     const int r0 = z0 >= 2 ? z0 - 2 : 0;            /* first row of step 1 */
     const int rs2 = z0 >= 1 ? z0 - 1 : 0;           /* first row of step 2 */
@@ -1230,17 +1275,15 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     auto fetch = [&](int ez, RowFetch<N> &r) {
         const int iz = ez * N + b;
         const long rowoff = (long)iz * ld + (long)exl * N;
-        const SEMSKS_GMEM float4 *v4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(U + 4 * rowoff);
-#pragma unroll
-        for (int a = 0; a < N; ++a) {
-            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (valid) t = v4[a];
-            r.u[a][0] = t.x; r.u[a][1] = t.y; r.u[a][2] = t.z; r.c[a] = __float_as_uint(t.w);
-        }
-        {
-            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (direct) t = v4[N];
-            r.un[0] = t.x; r.un[1] = t.y; r.un[2] = t.z; r.cn = __float_as_uint(t.w);
+        row_load<N>(U + 3 * rowoff, valid, r.u);
+        uint4 w4 = make_uint4(0u, 0u, 0u, 0u);
+        if (valid) w4 = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(W + rowoff);
+        r.w[0] = w4.x; r.w[1] = w4.y; r.w[2] = w4.z; r.w[3] = w4.w;
+        r.un[0] = r.un[1] = r.un[2] = 0.f;
+        r.wn = 0u;
+        if (direct) {
+            xyz_load(U, rowoff + N, r.un);
+            r.wn = W[rowoff + N];
         }
         r.rid = 0;
         if constexpr (!MIXED) r.rid = (int)cp(D.emat)[(long)ez * nex + ex_first];
@@ -1249,14 +1292,11 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     auto fetch_prev = [&](int ez, RowFetch<N> &r) {
         const int iz = ez * N + b;
         const long rowoff = (long)iz * ld + (long)exl * N;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) r.pv[a] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool on = valid && (b < N || ez == nez - 1);
         r.m[0] = r.m[1] = r.m[2] = r.m[3] = 0.f;
-        if (valid && (b < N || ez == nez - 1)) {
+        row_load<4>(UP + 3 * rowoff, on, r.pv);
+        if (on) {
             const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
-            const SEMSKS_GMEM float4 *s4 = reinterpret_cast<const SEMSKS_GMEM float4 *>(UP + 4 * rowoff);
-#pragma unroll
-            for (int a = 0; a < 4; ++a) r.pv[a] = s4[a];
             const bool inner = ex > 0 && iz != 0 && iz != nz - 1;
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
@@ -1270,28 +1310,27 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
        column) from u(t) = uu/wu, u(t-dt) = pv/pw, dt^2/M = m and the force;
        lanes with no edge node take the interior form directly */
     auto row_next = [&](int iz, const float (&f)[NP][3], const float (&uu)[NP][3], const uint32_t (&wu)[NP],
-                        const float (&pv)[NP][3], const uint32_t (&pw)[NP], const float (&m)[4],
-                        float (&nu)[NP][3], uint32_t (&nw)[NP]) {
+                        const float (&pv)[NP][3], const float (&m)[4], float (&nu)[NP][3], uint32_t (&nw)[NP]) {
         if (ex > 0 && iz != 0 && iz != nz - 1) {
 #pragma unroll
             for (int a = 0; a < N; ++a) {
                 float d[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    d[k] = delta_of(uu[a][k], wu[a], pv[a][k], pw[a], k, kahan) - m[a] * f[a][k];
+                    d[k] = delta_of(uu[a][k], wu[a], pv[a][k], k, kahan) - m[a] * f[a][k];
                     nu[a][k] = uu[a][k];
                 }
                 nw[a] = wu[a];
-                kahan3(kahan, nu[a], nw[a], d);
+                kahan3<true>(kahan, nu[a], nw[a], d);
             }
         } else {
 #pragma unroll
             for (int a = 0; a < N; ++a)
-                next_node(D, kahan, edge_index(ex * N + a, iz, nx, nz, N), m[a], f[a], uu[a], wu[a], pv[a], pw[a],
-                          nu[a], nw[a]);
+                next_node(D, kahan, edge_index(ex * N + a, iz, nx, nz, N), m[a], f[a], uu[a], wu[a], pv[a], nu[a],
+                          nw[a]);
         }
         if (last_x)
-            next_node(D, kahan, edge_index(nx - 1, iz, nx, nz, N), 0.f, f[N], uu[N], wu[N], pv[N], pw[N], nu[N], nw[N]);
+            next_node(D, kahan, edge_index(nx - 1, iz, nx, nz, N), 0.f, f[N], uu[N], wu[N], pv[N], nu[N], nw[N]);
     };
 
     /* assembles a row force: z from the carried top row of the element
@@ -1385,21 +1424,16 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
             assemble(f, CF1);
             if (valid && (b < N || top)) {
                 float pv[NP][3];
-                uint32_t pw[NP], wu[NP];
+                uint32_t wu[NP];
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
-                    pv[a][0] = cur.pv[a].x; pv[a][1] = cur.pv[a].y; pv[a][2] = cur.pv[a].z;
-                    pw[a] = __float_as_uint(cur.pv[a].w);
-                    wu[a] = cur.c[a];
+                    pv[a][0] = cur.pv[a][0]; pv[a][1] = cur.pv[a][1]; pv[a][2] = cur.pv[a][2];
+                    wu[a] = cur.w[a];
                 }
-                wu[N] = cur.cn;
+                wu[N] = cur.wn;
                 pv[N][0] = pv[N][1] = pv[N][2] = 0.f;
-                pw[N] = 0u;
-                if (last_x) {
-                    const float4 t = *reinterpret_cast<const SEMSKS_GMEM float4 *>(UP + 4 * ((long)iz * ld + nx - 1));
-                    pv[N][0] = t.x; pv[N][1] = t.y; pv[N][2] = t.z; pw[N] = __float_as_uint(t.w);
-                }
-                row_next(iz, f, u, wu, pv, pw, cur.m, v1, w1);
+                if (last_x) xyz_load(UP, (long)iz * ld + nx - 1, pv[N]);
+                row_next(iz, f, u, wu, pv, cur.m, v1, w1);
                 if (inj && (iz == nbl || (ex * N <= nbl && nbl <= ex * N + N))) {
 #pragma unroll
                     for (int a = 0; a < NP; ++a) {
@@ -1407,15 +1441,23 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
                         if ((a < N || last_x) && is_inject_node(ix, iz, nbl)) {
                             float d[3];
                             inject_delta(D, it + 1, ix, iz, u[a][0], u[a][2], d);
-                            kahan3(kahan, v1[a], w1[a], d);
+                            kahan3<false>(kahan, v1[a], w1[a], d);
                         }
                     }
                 }
                 if (own_x && ez >= z0 && ez < z0 + rows) {
-                    SEMSKS_GMEM float4 *o4 = reinterpret_cast<SEMSKS_GMEM float4 *>(U1 + 4 * ((long)iz * ld + (long)ex * N));
+                    const long q = (long)iz * ld + (long)ex * N;
+                    float o[4][3];
 #pragma unroll
-                    for (int a = 0; a < N; ++a) o4[a] = make_float4(v1[a][0], v1[a][1], v1[a][2], __uint_as_float(w1[a]));
-                    if (last_x) o4[N] = make_float4(v1[N][0], v1[N][1], v1[N][2], __uint_as_float(w1[N]));
+                    for (int a = 0; a < N; ++a)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) o[a][k] = v1[a][k];
+                    row_store4(U1 + 3 * q, o);
+                    *reinterpret_cast<SEMSKS_GMEM uint4 *>(W1 + q) = make_uint4(w1[0], w1[1], w1[2], w1[3]);
+                    if (last_x) {
+                        xyz_store(U1, q + N, v1[N]);
+                        W1[q + N] = w1[N];
+                    }
                 }
             }
             /* the next row's loads go out before the second step */
@@ -1451,19 +1493,25 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
             if (own_x && mine && r >= z0 && r < z0 + rows) {
                 const int iz = r * N + b;
                 float pv[NP][3], nu[NP][3];
-                uint32_t pw[NP], nw[NP];
+                uint32_t nw[NP];
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
                     pv[a][0] = prv.u[a][0]; pv[a][1] = prv.u[a][1]; pv[a][2] = prv.u[a][2];
-                    pw[a] = prv.c[a];
                 }
                 pv[N][0] = prv.un[0]; pv[N][1] = prv.un[1]; pv[N][2] = prv.un[2];
-                pw[N] = prv.cn;
-                row_next(iz, f, v1p, w1p, pv, pw, prv.m, nu, nw);
-                SEMSKS_GMEM float4 *o4 = reinterpret_cast<SEMSKS_GMEM float4 *>(U2 + 4 * ((long)iz * ld + (long)ex * N));
+                row_next(iz, f, v1p, w1p, pv, prv.m, nu, nw);
+                const long q = (long)iz * ld + (long)ex * N;
+                float o[4][3];
 #pragma unroll
-                for (int a = 0; a < N; ++a) o4[a] = make_float4(nu[a][0], nu[a][1], nu[a][2], __uint_as_float(nw[a]));
-                if (last_x) o4[N] = make_float4(nu[N][0], nu[N][1], nu[N][2], __uint_as_float(nw[N]));
+                for (int a = 0; a < N; ++a)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) o[a][k] = nu[a][k];
+                row_store4(U2 + 3 * q, o);
+                *reinterpret_cast<SEMSKS_GMEM uint4 *>(W2 + q) = make_uint4(nw[0], nw[1], nw[2], nw[3]);
+                if (last_x) {
+                    xyz_store(U2, q + N, nu[N]);
+                    W2[q + N] = nw[N];
+                }
             }
         }
         prv = cur;
@@ -2061,13 +2109,17 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
 #define ACHK(x) do { if ((x) != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "device allocation failed"); goto fail; } } while (0)
     {
         float *st_all = nullptr;
-        /* three levels of 16-byte records; four for an engine that may
-           take two steps per sweep */
+        /* three levels (four for an engine that may take two steps per
+           sweep), each an xyz plane and a compensation-word plane; every
+           plane starts on a 256-byte boundary (nodes is a multiple of 64) */
         d.nlev = e->opts.steps_per_sweep == 2 ? 4 : 3;
         const int nplanes = d.nlev * 4;
         ACHK(dalloc(s, &st_all, nodes * nplanes));
         ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->stream));
-        for (int l = 0; l < d.nlev; ++l) d.u[l] = st_all + 4 * l * nodes;
+        for (int l = 0; l < d.nlev; ++l) {
+            d.u[l] = st_all + 3 * l * nodes;
+            d.comp[l] = reinterpret_cast<uint32_t *>(st_all + 3 * d.nlev * nodes + l * nodes);
+        }
     }
     if (p->atten > 0) {   /* atten_taper (sem2d_psvsh.c:1725-1738), double then float */
         const int nh = p->neh * p->hpn;
@@ -2408,13 +2460,13 @@ semsks_status semsks_get_field(semsks_engine *e, int k, float *out)
     if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
     const Slot &s = e->slots[k];
     const int p = e->h_levels[k] % s.d.nlev;
-    std::vector<float> aos((size_t)s.ld * s.nz * 4);
+    std::vector<float> aos((size_t)s.ld * s.nz * 3);
     HIPCHK(hipMemcpyAsync(aos.data(), s.d.u[p], aos.size() * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     for (int c = 0; c < 3; ++c)
         for (int iz = 0; iz < s.nz; ++iz)
             for (int ix = 0; ix < s.nx; ++ix)
-                out[((size_t)c * s.nz + iz) * s.nx + ix] = aos[((size_t)iz * s.ld + ix) * 4 + c];
+                out[((size_t)c * s.nz + iz) * s.nx + ix] = aos[((size_t)iz * s.ld + ix) * 3 + c];
     return SEMSKS_OK;
 }
 
@@ -2424,16 +2476,16 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     const Slot &s = e->slots[k];
     const size_t nodes = (size_t)s.ld * s.nz;
     float *buf = nullptr;
-    HIPCHK(hipMalloc(&buf, nodes * 7 * sizeof(float) + 256));
-    HIPCHK(hipMemsetAsync(buf, 0, nodes * 7 * sizeof(float), e->stream));
+    HIPCHK(hipMalloc(&buf, nodes * 6 * sizeof(float) + 256));
+    HIPCHK(hipMemsetAsync(buf, 0, nodes * 6 * sizeof(float), e->stream));
     InstDesc d = s.d;
-    std::vector<float> aos(nodes * 4, 0.f);
+    std::vector<float> aos(nodes * 3, 0.f);
     for (int c = 0; c < 3; ++c)
         for (int iz = 0; iz < s.nz; ++iz)
             for (int ix = 0; ix < s.nx; ++ix)
-                aos[((size_t)iz * s.ld + ix) * 4 + c] = u[((size_t)c * s.nz + iz) * s.nx + ix];
+                aos[((size_t)iz * s.ld + ix) * 3 + c] = u[((size_t)c * s.nz + iz) * s.nx + ix];
     d.u[0] = buf;
-    for (int c = 0; c < 3; ++c) d.fout[c] = buf + (4 + c) * nodes;
+    for (int c = 0; c < 3; ++c) d.fout[c] = buf + (3 + c) * nodes;
     HIPCHK(hipMemcpyAsync(d.u[0], aos.data(), aos.size() * 4, hipMemcpyHostToDevice, e->stream));
     d.active = 1;
     InstDesc *dd = nullptr;

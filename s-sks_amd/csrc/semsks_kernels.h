@@ -2,13 +2,13 @@
  * semsks_kernels.h -- device data layout shared by the kernels and the engine.
  *
  * Layout in HBM, per instance (slot):
- *   u[2][3]   displacement, ping-pong pair of SoA planes (x, y, z); plane
- *             row stride ld floats (ld >= nx+3, multiple of 64) so that an
- *             element's first hpn columns are one aligned vector load
- *   del[3]    delta = u(t) - u(t-dt), in place
- *   cmp       Kahan compensation of u, in place: one 32-bit word per node
- *             holding three signed bytes in units of ulp(u)/256 (absent
- *             without Kahan)
+ *   u[l]      displacement of time level l (3 levels rotating, 4 for
+ *             two-step sweeps): x, y, z per node, 12 bytes; row stride ld
+ *             nodes (ld >= nx+3, multiple of 64), so rows start on 128-byte
+ *             lines
+ *   comp[l]   one 32-bit word per node: the Kahan compensations of the three
+ *             components at level l and at level l-1, signed 5-bit fields in
+ *             units of ulp(u)/32 (zero without Kahan)
  *   mat_id    16-bit material record id per node (same row stride)
  *   emat      16-bit record id per element when all its nodes share one
  *             record, 0xFFFF otherwise
@@ -36,10 +36,12 @@ struct InstDesc {
     int nex, nez, nx, nz, ld;
     int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active, nlev;
     /* displacement of nlev (3, or 4 for two-step sweeps) time levels,
-       level it in u[it % nlev]; one 16-byte record per node: x, y, z and
-       the packed Kahan compensation of the three components (bits): node q
-       at u[l][4q..4q+3] */
+       level it in plane it % nlev: u[l] holds x, y, z per node (node q at
+       u[l][3q..3q+2]), comp[l] one 32-bit word per node with the packed
+       Kahan compensations of the three components at that level (bits
+       0-14) and at the level before (bits 15-29) */
     float *u[4];
+    uint32_t *comp[4];
     float *fout[3];                 /* force-only mode output planes, else null */
     const uint16_t *mat_id;
     const uint16_t *emat;
