@@ -43,10 +43,12 @@
  * one-step kernel already keeps the vector ALUs ~2/3 busy, so one step per
  * launch is the default.
  *
- * State in HBM: three time levels of u rotating (level it in buffer
- * it % 3), one 16-byte record per node {x, y, z, packed Kahan
- * compensation}.  A step reads level it (with halos) and level it-1 and
- * writes level it+1.  An engine that takes two steps per sweep keeps four
+ * State in HBM: three time levels of u rotating (level it in plane
+ * it % 3), each an xyz plane (12 bytes per node) and a plane of 32-bit
+ * compensation words that hold the packed Kahan compensations of that level
+ * and of the level before.  A step reads the xyz of level it (with halos),
+ * the xyz of level it-1 and the words of level it, and writes the xyz and
+ * words of level it+1.  An engine that takes two steps per sweep keeps four
  * levels (it % 4) and also writes it+2, which then never overwrites a level
  * that a neighbouring tile may still read.  Rows are padded to a multiple
  * of 64 nodes.
