@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 import semsks
-from semsks import configs
+from semsks import configs, inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -71,3 +71,31 @@ def test_full_mesh_invariants(name):
     assert np.array_equal(fa.view(np.uint32), f1.view(np.uint32))
     d = fb.astype(np.float64) - 2.0 * f1.astype(np.float64)
     assert np.sqrt((d * d).sum() / (4.0 * (f1.astype(np.float64) ** 2).sum())) <= 1e-7
+
+
+def build_case(case):
+    b = configs.Built(case)
+    pr = b.model.problem
+    k0 = max(0, int(1.2 / case.f0 / pr.sdt) - 8)
+    with semsks.Engine() as e:
+        psv, sh = e.planewave(*b.planewave_args(k0 + NST + 3, p=0.05, psi=30.0), to_host=True)
+    return b, np.ascontiguousarray(psv[k0:]), np.ascontiguousarray(sh[k0:])
+
+
+def test_mixed_degrees_in_one_engine_match_solo():
+    """Instances of polynomial degree 3, 4 and 5, on meshes with more tiles
+    per degree than the persistent grid has waves (so every degree takes its
+    tiles from its own per-XCD queues), advance in one engine -- one launch
+    per degree and step -- and give the same fields as alone, bit for bit."""
+    print("case: hpn 3/4/5 meshes of 900x300/1000x300/600x300 elements, %d steps, mixed vs solo" % NST, flush=True)
+    cases = [inputs.Case(nex=900, nez=300, hpn=3, nst=NST, nrec=20, sh_zero=False),
+             inputs.Case(nex=1000, nez=300, hpn=4, nst=NST, nrec=20, sh_zero=False),
+             inputs.Case(nex=600, nez=300, hpn=5, nst=NST, nrec=20, sh_zero=False)]
+    built = [build_case(c) for c in cases]
+    probs = [problem(b, psv, sh) for b, psv, sh in built]
+    solo = [fields([p], share=False, tile_z=4)[0][0] for p in probs]
+    for f in solo:
+        assert np.isfinite(f).all() and np.abs(f).max() > 1e-3
+    mixed, _ = fields(probs, share=False, tile_z=4)
+    for a, b in zip(mixed, solo):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
