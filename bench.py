@@ -95,7 +95,8 @@ def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0):
     eng.advance(steps, sync=True)
     eng.sync()
     wall = time.perf_counter() - t0
-    return dict(throughput=npts * steps / wall, ms_per_step=1e3 * wall / steps, npts=npts, engine=eng, slots=slots)
+    return dict(throughput=npts * steps / wall, ms_per_step=1e3 * wall / steps, npts=npts, engine=eng, slots=slots,
+                tile_rows=eng.tile_rows())
 
 
 def run_serving(rate, duration, nex_range, nez_range, nst_range, budget_gb=200.0, max_slots=1024, seed=None):
@@ -199,8 +200,9 @@ def cpu_baseline(case_builder, steps: int, threads: int | None = None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=32,
+    ap.add_argument("--steps", type=int, default=48,
+                    help="timed steps (a multiple of 16 replays only the 16-step captured run)")
+    ap.add_argument("--warmup", type=int, default=48,
                     help="untimed steps first (the tile autotuner and the hipGraph capture run here)")
     ap.add_argument("--config", default="c3")
     ap.add_argument("--batch", type=int, default=64)
@@ -248,7 +250,8 @@ def main():
                             extra_steps=args.kernel_steps)
             k2, _ = kernel_time(r2["engine"], args.kernel_steps)
             r2["engine"].close()
-            log("  plain %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms" % (r2["throughput"], r2["ms_per_step"], k2))
+            log("  plain %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms, tiles of %d rows" % (
+                r2["throughput"], r2["ms_per_step"], k2, r2["tile_rows"]))
             if args.only == "plain":
                 return
         # batched: 64 waves on the C2 mesh with the isotropic model, one material
@@ -258,7 +261,8 @@ def main():
         r3 = run_engine(waves, args.steps, args.warmup, True, tile_z=args.tile_z, extra_steps=args.kernel_steps)
         k3, _ = kernel_time(r3["engine"], args.kernel_steps)
         r3["engine"].close()
-        log("  batched %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms" % (r3["throughput"], r3["ms_per_step"], k3))
+        log("  batched %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms, tiles of %d rows" % (
+            r3["throughput"], r3["ms_per_step"], k3, r3["tile_rows"]))
         # the same two legs on the 200x100 mesh (321,201 points per instance)
         r1p = r1b = None
         if args.only is None:
@@ -269,7 +273,8 @@ def main():
             w1 = [(b1, p, baz) for p, baz in configs.c3_waves()][: args.batch]
             r1b = run_engine(w1, args.steps, args.warmup, True, tile_z=args.tile_z)
             r1b["engine"].close()
-            log("  C1 plain %.3e, batched x%d %.3e pt-steps/s" % (r1p["throughput"], len(w1), r1b["throughput"]))
+            log("  C1 plain %.3e (tiles of %d rows), batched x%d %.3e pt-steps/s (%d rows)" % (
+                r1p["throughput"], r1p["tile_rows"], len(w1), r1b["throughput"], r1b["tile_rows"]))
         B = alg_bytes(9, len(waves))
         achieved = B * r3["npts"] / (k3 * 1e-3) / 1e9
         value, ms, npts, steps = r3["throughput"], r3["ms_per_step"], r3["npts"], args.steps
@@ -280,6 +285,7 @@ def main():
                "plain_value": r2["throughput"], "plain_ms_per_step": r2["ms_per_step"],
                "batched_over_plain": r3["throughput"] / r2["throughput"],
                "kernel_ms_batched": k3, "kernel_ms_plain": k2,
+               "tile_rows_batched": r3["tile_rows"], "tile_rows_plain": r2.get("tile_rows"),
                "bytes_per_point_step_alg": B, "kahan": True}
         if args.only is None and args.serve_window > 0:
             log("serving: Poisson 20/s for %.0f s, scaled C4-style requests" % args.serve_window)
@@ -294,6 +300,7 @@ def main():
                                                          cfg["precision"]["w"]["rel_l2_max"]))
         if r1p is not None:
             cfg.update({"c1_mesh_plain_value": r1p["throughput"], "c1_mesh_batched_value": r1b["throughput"],
+                    "c1_mesh_tile_rows_plain": r1p["tile_rows"], "c1_mesh_tile_rows_batched": r1b["tile_rows"],
                         "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
                         "c1_mesh_workload": "64 plane S waves on the 200x100-element mesh vs one"})
         cpu_case = lambda: configs.c2_case(nst=10)

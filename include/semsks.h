@@ -207,6 +207,10 @@ int semsks_last_timed_steps(semsks_engine *e);
    the two-step sweep is in use (semsks_engine_opts.steps_per_sweep = 2 and
    every instance allows it), else 1.  Valid after semsks_advance. */
 int semsks_sweep_steps(semsks_engine *e);
+/* Element rows per tile of the current tiling: the autotuned height once
+   the autotuner has run for this instance set, the engine's default before,
+   or semsks_engine_opts.tile_z when set. */
+int semsks_tile_rows(semsks_engine *e);
 
 /* ---------------------------------------------------------------------- */
 /* Serving: continuous batching of arriving requests                       */

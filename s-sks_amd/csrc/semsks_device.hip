@@ -1713,9 +1713,10 @@ struct semsks_engine {
     int tile_cap = 0;
     int tile_off[SEMSKS_MAXNP] = {0}; /* offset of each hpn's tiles */
     bool dirty = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_steps = 0;
+    /* captured steps: graph[i] holds 16 >> i steps (16, 8, 4, 2, 1), all
+       built for the same instance set (the graph_* fields below) */
+    hipGraph_t graph[5] = {};
+    hipGraphExec_t gexec[5] = {};
     unsigned hpn_mask = 0, graph_mask = 0;
     bool any_sponge = false, graph_sponge = false;
     /* two steps per sweep (step2_kernel) for the current instance set, and
@@ -1729,6 +1730,7 @@ struct semsks_engine {
     /* tile height in element rows: opts.tile_z when set, else the
        autotuner's choice for the current set of instances */
     int tz_cur = 8;
+    bool tuning = false;               /* the autotuner is trying heights */
     bool tuned = false;
     std::map<uint64_t, int> tz_cache;   /* instance-set signature -> tile height */
     /* device-resident plane-wave records waiting for the next admit */
@@ -1743,6 +1745,7 @@ static int tile_rows(const semsks_engine *e, int nez)
 }
 
 static void drop_graph(semsks_engine *e);
+static int default_tile_rows(const semsks_engine *e);
 
 /* What the best tile height depends on: degree mix, how many instances
    share the step and how many element columns and rows they have (both
@@ -1836,6 +1839,13 @@ static semsks_status upload_tables(semsks_engine *e)
             e->tz_cur = it->second;
             return upload_tables(e);
         }
+        if (!e->tuned && !e->tuning) {
+            const int d = default_tile_rows(e);
+            if (d != tzg) {
+                e->tz_cur = d;
+                return upload_tables(e);
+            }
+        }
     }
     return SEMSKS_OK;
 }
@@ -1874,6 +1884,35 @@ static int step2_blocks(const semsks_engine *e)
          per_cu < 1))
         per_cu = 2;
     return e->n_cu * per_cu;
+}
+
+/* Tile height before (or without) the autotuner: the tallest of 32, 16,
+   ..., 1 element rows that still gives every degree's launch at least two
+   tiles per wave of its persistent grid.  Many short tiles keep the grid of
+   a small mesh busy and its sweeps short (one 200x100-element instance:
+   0.020 ms per step at 1 row, 0.036 at 4); tall tiles read fewer halo rows
+   (64 instances on that mesh: 0.32 ms at 8-16 rows, 0.49 at 1). */
+static int default_tile_rows(const semsks_engine *e)
+{
+    int best = 32;
+    for (int n = 3; n <= 6; ++n) {
+        const bool two = e->two && n == 4;
+        const long waves = 4L * (two ? step2_blocks(e) : persistent_blocks<false>(e, n));
+        const int xc = two ? XC2 : xchunk_of(n);
+        bool any = false;
+        int tz = 1;
+        for (int c = 32; c >= 1; c /= 2) {
+            long tiles = 0;
+            for (const Slot &sl : e->slots)
+                if (sl.used && sl.meta.hpn == n) {
+                    any = true;
+                    tiles += (long)((sl.meta.nex + xc - 1) / xc) * ((sl.meta.nez + c - 1) / c);
+                }
+            if (tiles >= 2 * waves) { tz = c; break; }
+        }
+        if (any) best = std::min(best, tz);
+    }
+    return best;
 }
 
 /* one launch of the element-force kernel for the tiles of polynomial degree n */
@@ -1920,23 +1959,34 @@ static semsks_status launch_step(semsks_engine *e, int smax, hipEvent_t ta, hipE
 
 static void drop_graph(semsks_engine *e)
 {
-    if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
-    if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
-    e->graph_steps = 0;
+    for (int i = 0; i < 5; ++i) {
+        if (e->gexec[i]) { hipGraphExecDestroy(e->gexec[i]); e->gexec[i] = nullptr; }
+        if (e->graph[i]) { hipGraphDestroy(e->graph[i]); e->graph[i] = nullptr; }
+    }
 }
 
-static semsks_status build_graph(semsks_engine *e, int steps)
+/* the captured steps still match the instance set (tiles, degrees, sponge,
+   sweep mode)? */
+static bool graphs_current(const semsks_engine *e)
 {
-    drop_graph(e);
+    return e->graph_mask == e->hpn_mask && memcmp(e->graph_off, e->tile_off, sizeof e->graph_off) == 0 &&
+           e->graph_sponge == e->any_sponge && e->graph_two == e->two;
+}
+
+/* captures graph[i], 16 >> i steps (a multiple of the steps per launch) */
+static semsks_status build_graph(semsks_engine *e, int i)
+{
+    bool any = false;
+    for (int j = 0; j < 5; ++j) any = any || e->gexec[j];
+    if (any && !graphs_current(e)) drop_graph(e);
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    const int S = e->two ? 2 : 1;
+    const int S = e->two ? 2 : 1, steps = 16 >> i;
     for (int s = 0; s < steps; s += S) {
         semsks_status st = launch_step(e, S, nullptr, nullptr);
         if (st != SEMSKS_OK) { hipGraph_t g; hipStreamEndCapture(e->stream, &g); if (g) hipGraphDestroy(g); return st; }
     }
-    HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
-    HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
-    e->graph_steps = steps;
+    HIPCHK(hipStreamEndCapture(e->stream, &e->graph[i]));
+    HIPCHK(hipGraphInstantiate(&e->gexec[i], e->graph[i], nullptr, nullptr, 0));
     e->graph_mask = e->hpn_mask;
     e->graph_sponge = e->any_sponge;
     e->graph_two = e->two;
@@ -2013,7 +2063,7 @@ static int nt_sem_of(const semsks_problem *p) { return (int)((float)((p->nt - 3)
 size_t semsks_footprint(const semsks_problem *p, int shared)
 {
     const size_t nodes = (size_t)ld_of(p) * nz_of(p);
-    size_t b = nodes * 4 * (3 * 4);                 /* three levels of {x, y, z, packed compensation} */
+    size_t b = nodes * 4 * (3 * 4);                 /* three levels: xyz plane + compensation words */
     if (!shared) b += nodes * 2 + (size_t)p->nex * p->nez * 2 +
                       (size_t)p->n_mat * (64 + 4 * (p->hpn + 1) * (p->hpn + 1)) +
                       (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24;
@@ -2309,18 +2359,26 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     e->timed_steps = 0;
     int done = 0;
     /* autotuner: with tile_z unset, the first call that has the steps to
-       spare runs three real launches at each of four candidate tile
-       heights and keeps the height whose faster of the last two was
-       fastest (the tiling does not change the arithmetic, so these steps
-       are ordinary steps of the run).  Measured on the 2000x500 mesh: one
-       instance is fastest at 4 rows (many tiles per wave balance the
-       persistent grid), a 64-instance batch is insensitive to the height */
-    const int ncand = 4;
+       spare runs four real launches at each candidate tile height (six
+       for one step per launch, four for two-step sweeps) and keeps the
+       height whose fastest of the last three was fastest (the tiling does not
+       change the arithmetic, so these steps are ordinary steps of the run).
+       Until then, and in calls too short for it (the serving loop's
+       rounds), default_tile_rows picks the height.  Measured: one
+       instance on the 200x100 mesh is fastest at 1 row, on the 2000x500
+       mesh at 4 (many tiles per wave balance the persistent grid); a
+       64-instance batch is nearly insensitive to the height */
     const int S = e->two ? 2 : 1;     /* steps per launch */
+    static const int cand1[] = {1, 2, 4, 8, 16, 32}, cand2[] = {8, 16, 32, 64};
+    const int *cand = e->two ? cand2 : cand1;
+    const int ncand = e->two ? 4 : 6;
     if (e->opts.tile_z <= 0 && !e->tuned && nsteps >= 4 * ncand * S && sync != 2) {
-        static const int cand1[] = {4, 8, 16, 32}, cand2[] = {8, 16, 32, 64};
-        const int *cand = e->two ? cand2 : cand1;
-        while ((int)e->tev.size() < 6) {
+        struct TuneFlag {
+            semsks_engine *e;
+            ~TuneFlag() { e->tuning = false; }
+        } tune_flag{e};
+        e->tuning = true;
+        while ((int)e->tev.size() < 8) {
             hipEvent_t v;
             HIPCHK(hipEventCreate(&v));
             e->tev.push_back(v);
@@ -2331,20 +2389,20 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
             e->tz_cur = cand[c];
             semsks_status st = upload_tables(e);
             if (st) return st;
-            /* three launches; the faster of the last two counts */
-            for (int r = 0; r < 3; ++r) {
+            /* four launches; the fastest of the last three counts */
+            for (int r = 0; r < 4; ++r) {
                 st = launch_step(e, S, e->tev[2 * r], e->tev[2 * r + 1]);
                 if (st) return st;
             }
-            HIPCHK(hipEventSynchronize(e->tev[5]));
+            HIPCHK(hipEventSynchronize(e->tev[7]));
             float ms = 1e30f;
-            for (int r = 1; r < 3; ++r) {
+            for (int r = 1; r < 4; ++r) {
                 float m = 0.f;
                 HIPCHK(hipEventElapsedTime(&m, e->tev[2 * r], e->tev[2 * r + 1]));
                 ms = std::min(ms, m);
             }
             if (ms < best_ms) { best_ms = ms; best = cand[c]; }
-            done += 3 * S;
+            done += 4 * S;
         }
         e->tz_cur = best;
         semsks_status st = upload_tables(e);
@@ -2354,21 +2412,25 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
         drop_graph(e);
     }
     if (e->opts.use_graph && sync != 2) {
-        const int G = 16;
-        if (nsteps - done >= G) {
-            if (!e->gexec || e->graph_steps != G || e->graph_mask != e->hpn_mask ||
-                memcmp(e->graph_off, e->tile_off, sizeof e->graph_off) != 0 || e->graph_sponge != e->any_sponge ||
-                e->graph_two != e->two) {
-                semsks_status st = build_graph(e, G);
+        /* runs of 16 captured steps, the rest in captured runs of 8, 4, 2
+           and 1 (a two-step sweep's odd last step eagerly) */
+        if (!graphs_current(e)) drop_graph(e);
+        while (done < nsteps) {
+            const int rem = nsteps - done;
+            int i = 0;
+            while (i < 5 && ((16 >> i) > rem || (16 >> i) % S != 0)) ++i;
+            if (i == 5) {
+                semsks_status st = launch_step(e, rem, nullptr, nullptr);
+                if (st) return st;
+                done += rem;
+                continue;
+            }
+            if (!e->gexec[i]) {
+                semsks_status st = build_graph(e, i);
                 if (st) return st;
             }
-            for (; done + G <= nsteps; done += G) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
-        }
-        while (done < nsteps) {
-            const int s = std::min(S, nsteps - done);
-            semsks_status st = launch_step(e, s, nullptr, nullptr);
-            if (st) return st;
-            done += s;
+            HIPCHK(hipGraphLaunch(e->gexec[i], e->stream));
+            done += 16 >> i;
         }
     } else {
         /* eager launches with an event pair around every step-kernel launch
@@ -2429,6 +2491,12 @@ semsks_status semsks_sync(semsks_engine *e)
 int semsks_sweep_steps(semsks_engine *e)
 {
     return e && e->two ? 2 : 1;
+}
+
+int semsks_tile_rows(semsks_engine *e)
+{
+    if (!e) return 0;
+    return e->opts.tile_z > 0 ? e->opts.tile_z : e->tz_cur;
 }
 
 semsks_status semsks_slot_status(semsks_engine *e, int k, int *level, int *done)

@@ -29,7 +29,7 @@ EXPORTS = [
     "semsks_model_free", "semsks_write_seis", "semsks_last_error", "semsks_engine_create",
     "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_advance",
     "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_retire", "semsks_sync",
-    "semsks_last_timing", "semsks_last_timed_steps", "semsks_sweep_steps", "semsks_run_problem", "semsks_stiff",
+    "semsks_last_timing", "semsks_last_timed_steps", "semsks_sweep_steps", "semsks_tile_rows", "semsks_run_problem", "semsks_stiff",
     "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve", "semsks_uvw2sac",
 ]
 
@@ -105,6 +105,8 @@ def lib() -> C.CDLL:
     L.semsks_last_timed_steps.argtypes = [vp]
     L.semsks_sweep_steps.restype = C.c_int
     L.semsks_sweep_steps.argtypes = [vp]
+    L.semsks_tile_rows.restype = C.c_int
+    L.semsks_tile_rows.argtypes = [vp]
     L.semsks_run_problem.argtypes = [C.POINTER(Problem), vp]
     L.semsks_stiff.argtypes = [vp, C.c_int, vp, vp]
     L.semsks_get_setup.argtypes = [vp, C.c_int, vp, vp, vp]
@@ -302,6 +304,10 @@ class Engine:
         """Time steps per step-kernel launch for the current instances (2 in
         the two-step sweep, else 1)."""
         return int(lib().semsks_sweep_steps(self._h))
+
+    def tile_rows(self) -> int:
+        """Element rows per tile of the current tiling."""
+        return int(lib().semsks_tile_rows(self._h))
 
     def retire(self, slot: int):
         _check(lib().semsks_retire(self._h, slot))

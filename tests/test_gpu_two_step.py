@@ -90,3 +90,19 @@ def test_receiver_on_injection_column_falls_back(tmp_path, monkeypatch):
     t1, f1, s1 = run(m.problem, steps_per_sweep=1)
     t0, f0, s0 = run(m.problem, steps_per_sweep=2)
     assert s0 == 1 and same(t0, t1) and same(f0, f1)
+
+
+def test_captured_runs_equal_eager_steps(tmp_path, monkeypatch):
+    """Advances of odd lengths replay the captured runs of 16, 8, 4, 2 and 1
+    steps many times, with eager host work and synchronisation between the
+    calls; traces and field equal those of eager launches bit for bit."""
+    print("case: small, chunks 5/16/3/1/27/rest, captured vs eager", flush=True)
+    d = make_case(tmp_path, nst=240, psi_deg=20.0, sh_zero=False)
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    chunks = [5, 16, 3, 1, 27, 31, 2, 154]
+    assert sum(chunks) == m.problem.nst - 1
+    t0, f0, _ = run(m.problem, chunks=chunks, use_graph=0)
+    assert np.abs(t0).max() > 1e-3
+    t1, f1, _ = run(m.problem, chunks=chunks, use_graph=1)
+    assert same(t1, t0) and same(f1, f0)
