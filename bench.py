@@ -174,8 +174,10 @@ def kernel_time(eng, nsteps):
     return kern / max(n, 1), total / nsteps
 
 
-def cpu_baseline(case_builder, steps: int, threads: int | None = None):
-    """Faithful FP32 port (-O3) on the host: one instance of the mesh."""
+def cpu_baseline(case_builder, steps: int, threads: int | None = None, kind: str = "f32"):
+    """Faithful FP32 port on the host, one instance of the mesh: kind "f32"
+    is the -O3 build, "f32_O0" the build with the reference makefile's own
+    flags (cc -fopenmp, no -O)."""
     import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -187,7 +189,7 @@ def cpu_baseline(case_builder, steps: int, threads: int | None = None):
         c.nst = steps + 1
         c.write(d, nt=steps + 8)
         t0 = time.time()
-        o = pyoracle.Oracle(["par=run.par"], "f32", cwd=d)
+        o = pyoracle.Oracle(["par=run.par"], kind, cwd=d)
         setup = time.time() - t0
         t0 = time.perf_counter()
         o.run(steps + 1)
@@ -209,6 +211,8 @@ def main():
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-steps-o0", type=int, default=2,
+                    help="steps of the port built with the reference makefile's flags (0 = skip)")
     ap.add_argument("--serve-window", type=float, default=5.0,
                     help="seconds of Poisson arrivals in the serving leg of the default run (0 = skip)")
     ap.add_argument("--only", choices=["plain", "batched"], default=None,
@@ -377,6 +381,12 @@ def main():
                          "loops as the reference), 1 instance of the %d-point mesh, %d steps (%.1f s)"
                          % (cb["nglob"], cb["steps"], cb["wall"])}
         log("  cpu %.3e pt-steps/s" % cb["value"])
+        if args.cpu_steps_o0 > 0:
+            c0 = cpu_baseline(cpu_case, args.cpu_steps_o0, threads, kind="f32_O0")
+            cpu["makefile_flags"] = {"value": c0["value"], "cores": threads,
+                                     "sample": "the same port built with the reference makefile's flags "
+                                               "(cc -fopenmp, no -O), %d steps (%.1f s)" % (c0["steps"], c0["wall"])}
+            log("  cpu, makefile flags: %.3e pt-steps/s" % c0["value"])
 
     line = {
         "metric": METRIC, "value": value, "unit": "GLL point-steps/s", "n_gpus": 1, "steps": steps,
