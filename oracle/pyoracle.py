@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/sem_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg.  The product never loads it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check its result and by bench.py's cpu_baseline leg.  The product never loads
+it.
 """
 from __future__ import annotations
 

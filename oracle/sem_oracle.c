@@ -1,8 +1,8 @@
 /*
  * sem_oracle.c -- CPU restatement of the s-sks 2-D SEM time loop.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP product
- * path is compared against, and the timed CPU baseline.  Nothing in the
+ * TEST INFRASTRUCTURE ONLY.  The tests compare the HIP product path against
+ * this file, and the bench times it as the CPU baseline.  Nothing in the
  * product library links, loads or calls it.
  *
  * The file is compiled twice:

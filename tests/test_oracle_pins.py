@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the checker.
+"""Pins the CPU oracle before the GPU path is compared against it.
 
 Each oracle function is checked against something that does not restate it:
   * golden vectors written by the reference's own interpl.c / readmodel.c
