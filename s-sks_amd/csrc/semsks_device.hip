@@ -374,12 +374,10 @@ __device__ __forceinline__ uint32_t c_pack3(int q0, int q1, int q2)
     return ((uint32_t)q0 & 31u) | (((uint32_t)q1 & 31u) << 5) | (((uint32_t)q2 & 31u) << 10);
 }
 
-/* u += d for the three components.  w is the word of u's level; ADVANCE:
-   u becomes the next level, whose word takes the new fields and keeps u's
-   old ones as its previous-level fields; otherwise (an in-place change of
-   the same level, the interface injection) the previous-level fields stay.
-   Without compensation w stays 0. */
-template <bool ADVANCE>
+/* u += d for the three components as an in-place change of one level (the
+   interface injection): w is the word of u's level, its own fields take the
+   new compensations and the previous level's fields stay.  Without
+   compensation w stays 0.  (A step to the next level is advance3.) */
 __device__ __forceinline__ void kahan3(bool kahan, float (&u)[3], uint32_t &w, const float (&d)[3])
 {
     if (kahan) {
@@ -390,8 +388,7 @@ __device__ __forceinline__ void kahan3(bool kahan, float (&u)[3], uint32_t &w, c
             kahan_add(u[k], c, d[k]);
             q[k] = c_quant(c, u[k]);
         }
-        const uint32_t hi = ADVANCE ? (w & C_OWN) << 15 : (w & ~C_OWN);
-        w = c_pack3(q[0], q[1], q[2]) | hi;
+        w = c_pack3(q[0], q[1], q[2]) | (w & ~C_OWN);
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) u[k] = u[k] + d[k];
@@ -475,7 +472,7 @@ __global__ void inject_kernel(const InstDesc *insts, const int *levels, int *qct
         inject_delta(D, it, ix, iz, mp ? Up[3 * q] : 0.f, mp ? Up[3 * q + 2] : 0.f, d);
         float u[3] = {U[3 * q], U[3 * q + 1], U[3 * q + 2]};
         uint32_t w = W[q];
-        kahan3<false>(D.kahan != 0, u, w, d);
+        kahan3(D.kahan != 0, u, w, d);
         for (int k = 0; k < 3; ++k) U[3 * q + k] = u[k];
         W[q] = w;
     }
@@ -670,37 +667,54 @@ __device__ __forceinline__ float mdt2_of(const InstDesc &D, int mid, int ix, int
     return gp(D.mclass)[(long)mid * NC * NC + mass_class<N>(iz, D.nz - 1) * NC + mass_class<N>(ix, D.nx - 1)];
 }
 
-/* delta of the previous step from the compensated values of u(t) and
-   u(t-dt): (u - up) - (c - cp), c the compensation (true value u - c); w is
-   the word of level t, which holds both c and cp */
-__device__ __forceinline__ float delta_of(float u, uint32_t w, float up, int k, bool kahan)
+/* u(t+dt) of one node, in place: u = u(t) with its word w, up = u(t-dt).
+   The previous increment (u - up) - (c - cp) (c, cp the compensations of
+   levels t and t-dt, both in w; true value u - c), the step's increment
+   (interior: minus m f with m = dt^2/M; edge: ea * delta - eb * f, the
+   Lysmer rows), then the Kahan-compensated sum; w becomes the word of
+   level t+dt.  Each compensation is unpacked once. */
+__device__ __forceinline__ void advance3(bool kahan, bool edge, float (&u)[3], uint32_t &w, const float (&up)[3],
+                                         const float (&f)[3], float m, const float (&ea)[3], const float (&eb)[3])
 {
-    const float d = u - up;
-    return kahan ? d - (c_field(w, k, u) - c_field(w, k + 3, up)) : d;
+    if (!kahan) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float d = u[k] - up[k];
+            d = edge ? ea[k] * d - eb[k] * f[k] : d - m * f[k];
+            u[k] = u[k] + d;
+        }
+        w = 0u;
+        return;
+    }
+    int q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float c = c_field(w, k, u[k]);
+        float d = (u[k] - up[k]) - (c - c_field(w, k + 3, up[k]));
+        d = edge ? ea[k] * d - eb[k] * f[k] : d - m * f[k];
+        const float y = d - c;
+        const float t = u[k] + y;
+        q[k] = c_quant((t - u[k]) - y, t);
+        u[k] = t;
+    }
+    w = c_pack3(q[0], q[1], q[2]) | (w & C_OWN) << 15;
 }
 
-/* u(t+dt) of one node: delta from u(t) (u, word w) and u(t-dt) (up), then
-   interior nodes (eidx < 0) by dt^2/M = m, edge nodes by their Lysmer
-   factors, and the Kahan-compensated sum (word wn of level t+dt) */
+/* u(t+dt) of one node: interior nodes (eidx < 0) by dt^2/M = m, edge nodes
+   by their Lysmer factors (word wn of level t+dt) */
 __device__ __forceinline__ void next_node(const InstDesc &D, bool kahan, int eidx, float m, const float (&f)[3],
                                           const float (&uc)[3], uint32_t w, const float (&up)[3],
                                           float (&nu)[3], uint32_t &wn)
 {
-    float d[3], ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
-    d[0] = delta_of(uc[0], w, up[0], 0, kahan);
-    d[1] = delta_of(uc[1], w, up[1], 1, kahan);
-    d[2] = delta_of(uc[2], w, up[2], 2, kahan);
+    float ea[3] = {1.f, 1.f, 1.f}, eb[3] = {0.f, 0.f, 0.f};
     if (eidx >= 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) { ea[k] = gp(D.edge_a)[eidx * 3 + k]; eb[k] = gp(D.edge_b)[eidx * 3 + k]; }
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        d[k] = eidx < 0 ? d[k] - m * f[k] : ea[k] * d[k] - eb[k] * f[k];
-        nu[k] = uc[k];
-    }
+    for (int k = 0; k < 3; ++k) nu[k] = uc[k];
     wn = w;
-    kahan3<true>(kahan, nu, wn, d);
+    advance3(kahan, eidx >= 0, nu, wn, up, f, m, ea, eb);
 }
 
 /* one node: delta, Kahan-compensated u, edge rows by their Lysmer factors;
@@ -1100,16 +1114,13 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
             } else if (N == 4 && vec) {
                 float nv[4][3];
                 uint32_t nw[4];
+                const float one[3] = {1.f, 1.f, 1.f}, zero[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
-                    float d[3];
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        d[k] = delta_of(u[a][k], cur.w[a], cur.pv[a][k], k, kahan) - cur.m[a] * f[a][k];
-                        nv[a][k] = u[a][k];
-                    }
+                    for (int k = 0; k < 3; ++k) nv[a][k] = u[a][k];
                     nw[a] = cur.w[a];
-                    kahan3<true>(kahan, nv[a], nw[a], d);
+                    advance3(kahan, false, nv[a], nw[a], cur.pv[a], f[a], cur.m[a], one, zero);
                 }
                 row_store4(gp(D.u[pn]) + 3 * q0, nv);
                 *reinterpret_cast<SEMSKS_GMEM uint4 *>(gp(D.comp[pn]) + q0) = make_uint4(nw[0], nw[1], nw[2], nw[3]);
@@ -1314,16 +1325,13 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     auto row_next = [&](int iz, const float (&f)[NP][3], const float (&uu)[NP][3], const uint32_t (&wu)[NP],
                         const float (&pv)[NP][3], const float (&m)[4], float (&nu)[NP][3], uint32_t (&nw)[NP]) {
         if (ex > 0 && iz != 0 && iz != nz - 1) {
+            const float one[3] = {1.f, 1.f, 1.f}, zero[3] = {0.f, 0.f, 0.f};
 #pragma unroll
             for (int a = 0; a < N; ++a) {
-                float d[3];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    d[k] = delta_of(uu[a][k], wu[a], pv[a][k], k, kahan) - m[a] * f[a][k];
-                    nu[a][k] = uu[a][k];
-                }
+                for (int k = 0; k < 3; ++k) nu[a][k] = uu[a][k];
                 nw[a] = wu[a];
-                kahan3<true>(kahan, nu[a], nw[a], d);
+                advance3(kahan, false, nu[a], nw[a], pv[a], f[a], m[a], one, zero);
             }
         } else {
 #pragma unroll
@@ -1443,7 +1451,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
                         if ((a < N || last_x) && is_inject_node(ix, iz, nbl)) {
                             float d[3];
                             inject_delta(D, it + 1, ix, iz, u[a][0], u[a][2], d);
-                            kahan3<false>(kahan, v1[a], w1[a], d);
+                            kahan3(kahan, v1[a], w1[a], d);
                         }
                     }
                 }
