@@ -187,7 +187,9 @@ semsks_status semsks_slot_status(semsks_engine *e, int slot, int *level, int *do
 semsks_status semsks_get_traces(semsks_engine *e, int slot, int it0, int n, float *out);
 
 /* Copies the current displacement field of a slot to host as three SoA
-   planes [3][nz][nx] (u_x, u_y, u_z). */
+   planes [3][nz][nx] (u_x, u_y, u_z).  Before the last level, the
+   interface row and column already hold the injection of the next step
+   (GRT_Interface4/3 act on u(t) in place, after its traces are recorded). */
 semsks_status semsks_get_field(semsks_engine *e, int slot, float *out);
 
 /* Frees a slot (its device memory returns to the budget). */

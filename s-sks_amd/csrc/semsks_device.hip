@@ -2,13 +2,15 @@
  * semsks_device.hip -- gfx950 kernels and the engine behind include/semsks.h.
  *
  * One time step of one instance (semsyn/src/sem2d_psvsh.c:418-512):
- *   inject_kernel  GRT_Interface4 / GRT_Interface3 (sem2d_psvsh.c:2076-2105,
- *                  sem2dMP20140816.c:2001-2037) on the interface row/column
  *   step_kernel    Stiff_Matrix (sem2d_psvsh.c:1173-1256: dUj_di_2D, Stress_ij,
  *                  F_ik, Grid_Stiff) fused with Time_diff_hybrid + Absorbs
  *                  (sem2d_psvsh.c:1004-1029, 1834-1889) and the level rotation
- *                  (Copy_Displacement x2, :468-469, replaced by ping-pong)
- *   record_kernel  kirrecord_zhl (sem2d_psvsh.c:2116-2165)
+ *                  (Copy_Displacement x2, :468-469, replaced by rotating planes)
+ *   post_kernel    kirrecord_zhl (sem2d_psvsh.c:2116-2165) on the new level,
+ *                  then GRT_Interface4 / GRT_Interface3 (sem2d_psvsh.c:
+ *                  2076-2105, sem2dMP20140816.c:2001-2037) on the interface
+ *                  row/column of that level, for the next step (level 0 gets
+ *                  its injection at admission, inject0_kernel)
  *
  * The update is written in increment form: with delta_n = u_{n+1} - u_n,
  *   interior : delta_n = delta_{n-1} - (dt^2/M) f(u_n)
@@ -449,33 +451,40 @@ __device__ __forceinline__ bool is_inject_node(int ix, int iz, int nbl)
     return (iz == nbl && ix >= nbl) || (ix == nbl && iz > nbl);
 }
 
-__global__ void inject_kernel(const InstDesc *insts, const int *levels, int *qctr)
+/* The interface injection of one node, in place on level it (plane p):
+   interface index i runs over the bottom row iz = nbl (ix >= nbl, the
+   corner included) and then the left column ix = nbl (iz > nbl). */
+__device__ __forceinline__ void inject_node(const InstDesc &D, int it, int i, int nb)
 {
-    /* the step kernel that follows starts its tile queues at zero */
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < SEMSKS_QCOUNTERS) qctr[threadIdx.x * SEMSKS_QSTRIDE] = 0;
-    const InstDesc &D = insts[blockIdx.y];
-    if (!D.active) return;
-    const int it = levels[blockIdx.y];
-    if (it >= D.nst - 1) return;
-    if (it + 1 >= D.nt_sem) return;
-    const int L = D.nlev, p = it % L, nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld;
+    const int L = D.nlev, p = it % L, nbl = D.neh * D.hpn, ld = D.ld;
+    const int ix = i < nb ? nbl + i : nbl;
+    const int iz = i < nb ? nbl : nbl + 1 + (i - nb);
+    const long q = (long)iz * ld + ix;
     const bool mp = D.solver == SEMSKS_SOLVER_PSV_MP;
     float *U = D.u[p];
-    uint32_t *W = D.comp[p];
     const float *Up = D.u[(p + L - 1) % L];
-    const int nb = nx - nbl, nl = nz - nbl - 1;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb + nl; i += gridDim.x * blockDim.x) {
-        const int ix = i < nb ? nbl + i : nbl;
-        const int iz = i < nb ? nbl : nbl + 1 + (i - nb);
-        const long q = (long)iz * ld + ix;
-        float d[3];
-        inject_delta(D, it, ix, iz, mp ? Up[3 * q] : 0.f, mp ? Up[3 * q + 2] : 0.f, d);
-        float u[3] = {U[3 * q], U[3 * q + 1], U[3 * q + 2]};
-        uint32_t w = W[q];
-        kahan3(D.kahan != 0, u, w, d);
-        for (int k = 0; k < 3; ++k) U[3 * q + k] = u[k];
-        W[q] = w;
-    }
+    float d[3];
+    inject_delta(D, it, ix, iz, mp ? Up[3 * q] : 0.f, mp ? Up[3 * q + 2] : 0.f, d);
+    float u[3] = {U[3 * q], U[3 * q + 1], U[3 * q + 2]};
+    uint32_t w = D.comp[p][q];
+    kahan3(D.kahan != 0, u, w, d);
+    for (int k = 0; k < 3; ++k) U[3 * q + k] = u[k];
+    D.comp[p][q] = w;
+}
+
+/* does level it get an injection before the step that leaves it? */
+__device__ __forceinline__ bool injects(const InstDesc &D, int it)
+{
+    return it < D.nst - 1 && it + 1 < D.nt_sem;
+}
+
+/* the injection of level 0 at admission, before the instance's first step
+   (every later level gets its injection in post_kernel) */
+__global__ void inject0_kernel(InstDesc D)
+{
+    if (!injects(D, 0)) return;
+    const int nbl = D.neh * D.hpn, nb = D.nx - nbl, nl = D.nz - nbl - 1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb + nl; i += gridDim.x * blockDim.x) inject_node(D, 0, i, nb);
 }
 
 /* atten_wave2 (sem2d_psvsh.c:1780-1823): after the update, u(t+dt) on the
@@ -516,27 +525,62 @@ __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
     }
 }
 
-/* kirrecord_zhl: u to seis.u, -w to seis.w, v to seis.v for each level
-   the step launch produced (up to smax), then advance the level */
-__global__ void record_kernel(const InstDesc *insts, int *levels, int smax)
+/* kirrecord_zhl: u to seis.u, -w to seis.w, v to seis.v */
+__device__ __forceinline__ void record_one(const InstDesc &D, int lev, int r)
 {
-    const InstDesc &D = insts[blockIdx.x];
+    const int nt = D.ntrace;
+    const float *U = D.u[lev % D.nlev];
+    float *tr = D.traces + (long)lev * 3 * nt;
+    const long q = D.rec[r];
+    tr[r] = U[3 * q];
+    tr[nt + r] = -U[3 * q + 2];
+    tr[2 * nt + r] = U[3 * q + 1];
+}
+
+/* After a step launch, per slot (blockIdx.y), grid-wide over blockIdx.x:
+   records every level the launch produced (up to smax), then applies the
+   injection that the next step needs to the newest level in place
+   (GRT_Interface4 / GRT_Interface3, sem2d_psvsh.c:2076-2105,
+   sem2dMP20140816.c:2001-2037, which the reference applies to u(t) at the
+   top of the next iteration, after kirrecord_zhl has written it), and
+   advances the slot's level.  A receiver on an interface node is recorded
+   by the thread that injects that node, before the injection
+   (D.inj_rec[i]: first receiver at interface node i, D.rec_next[r]: next
+   one at the same node, -1 at the end of a list, -2 for a receiver off the
+   interface).  The last block of a slot to finish advances its level
+   (pctr counts the finished blocks).  It also clears the step kernel's
+   tile-queue counters for the next launch. */
+__global__ void post_kernel(const InstDesc *insts, int *levels, int smax, int *qctr, int *pctr)
+{
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < SEMSKS_QCOUNTERS) qctr[threadIdx.x * SEMSKS_QSTRIDE] = 0;
+    const int s = blockIdx.y;
+    const InstDesc &D = insts[s];
     if (!D.active) return;
-    const int it = levels[blockIdx.x];
+    const int it = levels[s];
     if (it >= D.nst - 1) return;
-    const int adv = min(smax, D.nst - 1 - it), nt = D.ntrace;
-    for (int l = 1; l <= adv; ++l) {
-        const int lev = it + l, p = lev % D.nlev;
-        float *tr = D.traces + (long)lev * 3 * nt;
-        for (int r = threadIdx.x; r < nt; r += blockDim.x) {
-            long q = D.rec[r];
-            tr[r] = D.u[p][3 * q];
-            tr[nt + r] = -D.u[p][3 * q + 2];
-            tr[2 * nt + r] = D.u[p][3 * q + 1];
+    const int adv = min(smax, D.nst - 1 - it), lev = it + adv, nt = D.ntrace;
+    const bool inj = injects(D, lev);
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+    for (int l = 1; l <= adv; ++l)
+        for (int r = tid; r < nt; r += nth)
+            if (!(inj && l == adv && D.rec_next[r] != -2)) record_one(D, it + l, r);
+    if (inj) {
+        const int nbl = D.neh * D.hpn, nb = D.nx - nbl, nl = D.nz - nbl - 1;
+        for (int i = tid; i < nb + nl; i += nth) {
+            for (int r = D.inj_rec[i]; r >= 0; r = D.rec_next[r]) record_one(D, lev, r);
+            inject_node(D, lev, i, nb);
         }
     }
+    /* every thread of the block has used the level it read; no data goes
+       from one block to another here (the next launch sees this kernel's
+       stores), so the count needs no fence */
     __syncthreads();
-    if (threadIdx.x == 0) levels[blockIdx.x] = it + adv;
+    if (threadIdx.x == 0) {
+        if (atomicAdd(pctr + s, 1) == (int)gridDim.x - 1) {
+            levels[s] = lev;
+            pctr[s] = 0;
+        }
+    }
 }
 
 __global__ void record0_kernel(InstDesc D)
@@ -1181,7 +1225,7 @@ __device__ __forceinline__ void step_tile(const InstDesc *__restrict__ insts, co
    their shared edge -- run at the same time under the same L2.
    Time steps with more tiles than waves (qctr != nullptr): each run is a
    queue with its counter in qctr[q * SEMSKS_QSTRIDE] (zero at launch:
-   inject_kernel clears them).  A wave takes the next tile of its own XCD's
+   post_kernel clears them).  A wave takes the next tile of its own XCD's
    queue, which keeps neighbours together however far the waves drift
    apart; a wave whose queue is empty takes from the next queues in turn
    and leaves once all NXCD are exhausted.  Otherwise the waves of an XCD
@@ -1241,7 +1285,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : 2) void step_kernel(const InstDes
    the second step; the top node row of an element comes from the lane that
    holds it as the bottom row of the element above.  Before the second
    step, u(t+dt) on the interface row/column gets the injection of level
-   it+1, exactly as inject_kernel would apply it in place.  Arithmetic is
+   it+1, exactly as post_kernel would apply it in place.  Arithmetic is
    the same, node by node, as two one-step launches.  two = false: only the
    first step (an instance with one step left). */
 constexpr int XC2 = 9;
@@ -1270,7 +1314,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     const int ex_first = x0 >= 2 ? x0 - 2 : 0;
     const RowCoef<N> rc = row_coef<N>(b);
     const int nbl = D.neh * N;
-    const bool inj = two && it + 2 < D.nt_sem;   /* inject_kernel's test for level it+1 */
+    const bool inj = two && injects(D, it + 1);
 
     const SEMSKS_GMEM float *U = gp(D.u[p]);
     const SEMSKS_GMEM float *UP = gp(D.u[(p + 3) % 4]);
@@ -1717,6 +1761,7 @@ struct semsks_engine {
     int4 *d_tiles = nullptr;
     int *d_ntiles = nullptr;          /* [7]: per hpn 3..6 (index hpn) */
     int *d_qctr = nullptr;            /* tile-queue counters, SEMSKS_QCOUNTERS lines */
+    int *d_pctr = nullptr;            /* post_kernel: finished blocks per slot */
     std::vector<int> h_levels;
     int tile_cap = 0;
     int tile_off[SEMSKS_MAXNP] = {0}; /* offset of each hpn's tiles */
@@ -1930,8 +1975,9 @@ static void launch_tiles(const semsks_engine *e, int n, const InstDesc *desc, co
 {
     const int nb = persistent_blocks<FORCE_ONLY>(e, n);
     hipStream_t st = e->stream;
-    /* time steps take tiles from the queues (cleared by inject_kernel just
-       before); the force-only launch deals them statically */
+    /* time steps take tiles from the queues (cleared by the post_kernel of
+       the launch before, or at engine creation); the force-only launch
+       deals them statically */
     int *q = FORCE_ONLY ? nullptr : e->d_qctr + n * SEMSKS_NQ * SEMSKS_QSTRIDE;
     switch (n) {
     case 3: hipLaunchKernelGGL((step_kernel<3, FORCE_ONLY>), dim3(nb), dim3(256), 0, st, desc, levels, tl, nt, q); break;
@@ -1947,7 +1993,6 @@ static semsks_status launch_step(semsks_engine *e, int smax, hipEvent_t ta, hipE
 {
     const int nslots = (int)e->slots.size();
     if (!e->two) smax = 1;
-    hipLaunchKernelGGL(inject_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels, e->d_qctr);
     if (ta) HIPCHK(hipEventRecord(ta, e->stream));
     if (e->two) {
         if (e->hpn_mask & (1u << 4))
@@ -1960,7 +2005,8 @@ static semsks_status launch_step(semsks_engine *e, int smax, hipEvent_t ta, hipE
     }
     if (tb) HIPCHK(hipEventRecord(tb, e->stream));
     if (e->any_sponge) hipLaunchKernelGGL(sponge_kernel, dim3(64, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
-    hipLaunchKernelGGL(record_kernel, dim3(nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels, smax);
+    hipLaunchKernelGGL(post_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels, smax, e->d_qctr,
+                       e->d_pctr);
     HIPCHK(hipGetLastError());
     return SEMSKS_OK;
 }
@@ -2031,6 +2077,8 @@ semsks_status semsks_engine_create(const semsks_engine_opts *opts, semsks_engine
     HIPCHK(hipMalloc(&e->d_ntiles, SEMSKS_MAXNP * sizeof(int)));
     HIPCHK(hipMalloc(&e->d_qctr, SEMSKS_QCOUNTERS * SEMSKS_QSTRIDE * sizeof(int)));
     HIPCHK(hipMemset(e->d_qctr, 0, SEMSKS_QCOUNTERS * SEMSKS_QSTRIDE * sizeof(int)));
+    HIPCHK(hipMalloc(&e->d_pctr, e->slots.size() * sizeof(int)));
+    HIPCHK(hipMemset(e->d_pctr, 0, e->slots.size() * sizeof(int)));
     std::vector<int> z(e->slots.size(), 0);
     HIPCHK(hipMemcpy(e->d_levels, z.data(), z.size() * sizeof(int), hipMemcpyHostToDevice));
     semsks_status st = upload_tables(e);
@@ -2056,7 +2104,7 @@ void semsks_engine_destroy(semsks_engine *e)
     for (size_t k = 0; k < e->slots.size(); ++k) if (e->slots[k].used) free_slot(e, (int)k);
     if (e->pend_psv) hipFree(e->pend_psv);
     if (e->pend_sh) hipFree(e->pend_sh);
-    hipFree(e->d_desc); hipFree(e->d_levels); hipFree(e->d_tiles); hipFree(e->d_ntiles); hipFree(e->d_qctr);
+    hipFree(e->d_desc); hipFree(e->d_levels); hipFree(e->d_tiles); hipFree(e->d_ntiles); hipFree(e->d_qctr); hipFree(e->d_pctr);
     for (auto &v : e->ev) if (v) hipEventDestroy(v);
     for (auto &v : e->tev) hipEventDestroy(v);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -2318,10 +2366,32 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         ACHK(hipMemcpyAsync(rec, off.data(), off.size() * 4, hipMemcpyHostToDevice, e->stream));
         ACHK(hipMemsetAsync(tr, 0, (size_t)p->nst * 3 * std::max(p->ntrace, 1) * 4, e->stream));
         d.rec = rec; d.traces = tr;
+        /* receivers on interface nodes, recorded by post_kernel's thread for
+           that node before it injects: per interface node the first such
+           receiver, per receiver the next one at its node (-1 ends a list,
+           -2 marks a receiver off the interface) */
+        const int nbl = p->neh * p->hpn, nb = std::max(s.nx - nbl, 0), nl = std::max(s.nz - nbl - 1, 0);
+        std::vector<int> head(std::max(nb + nl, 1), -1), next(std::max(p->ntrace, 1), -2);
+        for (int r = p->ntrace - 1; r >= 0; --r) {
+            const int ix = p->rec_ix[r], iz = p->rec_iz[r];
+            int i = -1;
+            if (iz == nbl && ix >= nbl && ix < s.nx) i = ix - nbl;
+            else if (ix == nbl && iz > nbl && iz < s.nz) i = nb + (iz - nbl - 1);
+            if (i < 0) continue;
+            next[r] = head[i];
+            head[i] = r;
+        }
+        int *dhead = nullptr, *dnext = nullptr;
+        ACHK(dalloc(s, &dhead, head.size()));
+        ACHK(dalloc(s, &dnext, next.size()));
+        ACHK(hipMemcpyAsync(dhead, head.data(), head.size() * 4, hipMemcpyHostToDevice, e->stream));
+        ACHK(hipMemcpyAsync(dnext, next.data(), next.size() * 4, hipMemcpyHostToDevice, e->stream));
+        d.inj_rec = dhead; d.rec_next = dnext;
     }
-    /* level 0 */
+    /* level 0: its state, its record, then the injection of the first step */
     hipLaunchKernelGGL(init_state_kernel, dim3(64), dim3(256), 0, e->stream, d);
     if (p->ntrace > 0) hipLaunchKernelGGL(record0_kernel, dim3(1), dim3(256), 0, e->stream, d);
+    hipLaunchKernelGGL(inject0_kernel, dim3(16), dim3(256), 0, e->stream, d);
     {
         int zero = 0;
         ACHK(hipMemcpyAsync(e->d_levels + k, &zero, sizeof zero, hipMemcpyHostToDevice, e->stream));

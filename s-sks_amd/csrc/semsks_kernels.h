@@ -50,6 +50,9 @@ struct InstDesc {
     const float *edge_a, *edge_b;   /* [n_edge][3] */
     const float *G, *Gv;            /* [nt_sem][2(nx+nz)], [nt_sem][nx+nz] */
     const int *rec;                 /* node offsets iz*ld+ix */
+    const int *inj_rec;             /* per interface node: first receiver on it, or -1 */
+    const int *rec_next;            /* per receiver: next one on its interface node (-1 at the
+                                       end), -2 if it is not on an interface node */
     const float *taper;             /* sponge factors [nh] (atten > 0) */
     int atten, nh;
     float *traces;

@@ -80,16 +80,25 @@ def test_two_step_uneven_batch(tmp_path, monkeypatch):
 def test_receiver_on_injection_column_falls_back(tmp_path, monkeypatch):
     """A receiver on the interface column would record the first level of a
     sweep after the next step's injection, so such an instance runs one step
-    per launch."""
+    per launch.  One step per launch records it before the injection, as
+    kirrecord_zhl does before the next GRT_Interface4: its trace, and the
+    others, match the FP64 oracle."""
     print("case: small, receiver at ix = nbl, fallback to one step", flush=True)
-    d = make_case(tmp_path, nst=120)
+    d = make_case(tmp_path, nst=500)
     monkeypatch.chdir(d)
-    m = semsks.Model(["par=run.par", "get_rec=0", "xbeg=2.75", "idxplot=3", "xtrace=5"])
-    ix, _ = m.receivers()
-    assert ix[0] == 4                                 # nbl = neh * hpn
+    args = ["par=run.par", "get_rec=0", "xbeg=2.75", "idxplot=3", "xtrace=5"]
+    m = semsks.Model(args)
+    ix, iz = m.receivers()
+    assert ix[0] == 4 and iz[0] > 4                   # nbl = neh * hpn: the left interface column
     t1, f1, s1 = run(m.problem, steps_per_sweep=1)
     t0, f0, s0 = run(m.problem, steps_per_sweep=2)
     assert s0 == 1 and same(t0, t1) and same(f0, f1)
+    import pyoracle
+    ref = pyoracle.Oracle(args, "f64", cwd=d).run()
+    for k in (0, 1):
+        a, b = t1[:, k, 0].astype(np.float64), ref[:, k, 0]
+        assert np.abs(b).max() > 1e-3
+        assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b), k
 
 
 def test_captured_runs_equal_eager_steps(tmp_path, monkeypatch):
