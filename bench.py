@@ -21,9 +21,11 @@ serving     = continuous batching (semsks_serve) of Poisson-arriving requests:
               300 s of full-size ones)
 traffic     = measured HBM bytes per launch, from the rocprofv3 summary in
               profiles/ for this workload
-precision   = the 200x100-element case's 100 seismograms, FP32 on the GPU
-              against the FP64 oracle's (a committed fixture): rel-L2 and
-              peak misfit, worst and median, per component
+precision   = the 200x100-element case's 100 seismograms (P-SV) and the
+              150 of the C2 model on that mesh (tilted-axis anisotropy,
+              coupled P-SV+SH), FP32 on the GPU against the FP64 oracle's
+              (committed fixtures): rel-L2 and peak misfit, worst and
+              median, per component
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3|c2|c1b|c5]
 """
@@ -131,16 +133,14 @@ def run_serving(rate, duration, nex_range, nez_range, nst_range, budget_gb=200.0
             "steps": "%d..%d" % nst_range}
 
 
-def precision_c1():
-    """FP32 seismograms of the 200x100-element case (4000 steps, 50
-    receivers) against the FP64 oracle's, which tests/golden/make_c1_fp64.py
-    wrote to tests/golden/c1_fp64_traces.npz: rel-L2 and peak misfit of every
-    seismogram, worst and median over the receivers, per component."""
+def precision_vs_fixture(case, fixture, label):
+    """FP32 GPU seismograms of a case against the FP64 oracle's, kept in a
+    fixture under tests/golden/ (written by the make_*_fp64.py script beside
+    it): rel-L2 and peak misfit of every seismogram, worst and median over
+    the receivers, per component."""
     import tempfile
     import semsks
-    from semsks import configs
-    ref = np.load(os.path.join(ROOT, "tests", "golden", "c1_fp64_traces.npz"))
-    case = configs.c1_case()
+    ref = np.load(os.path.join(ROOT, "tests", "golden", fixture))
     old = os.getcwd()
     with tempfile.TemporaryDirectory() as d:
         case.write(d)
@@ -153,17 +153,38 @@ def precision_c1():
                 tr = e.traces(s)
         finally:
             os.chdir(old)
-    out = {"case": "200x100 elements, 3 flat isotropic layers, plane SV p=0.06, 4000 steps, 50 receivers, P-SV",
-           "reference": "FP64 oracle (tests/golden/c1_fp64_traces.npz)", "seismograms": int(tr.shape[2]) * 2}
-    for name, k in (("u", 0), ("w", 1)):
+    out = {"case": label, "reference": "FP64 oracle (tests/golden/%s)" % fixture, "seismograms": 0}
+    for name, k, key in (("u", 0, "u"), ("w", 1, "mw"), ("v", 2, "v")):
+        if key not in ref.files:
+            out[name] = "zero in both (P-SV solver)" if np.abs(tr[:, k]).max() == 0 else "nonzero"
+            continue
         a = tr[:, k].astype(np.float64)
-        b = ref["u" if k == 0 else "mw"].astype(np.float64)
+        b = ref[key].astype(np.float64)
         l2 = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
         pk = np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)
         out[name] = {"rel_l2_max": float(l2.max()), "rel_l2_median": float(np.median(l2)),
                      "peak_misfit_max": float(pk.max()), "peak_misfit_median": float(np.median(pk))}
-    out["v"] = "zero in both (P-SV solver)" if np.abs(tr[:, 2]).max() == 0 else "nonzero"
+        out["seismograms"] += int(tr.shape[2])
     return out
+
+
+def precision_c1():
+    """The 200x100-element case (4000 steps, 50 receivers, P-SV) against
+    tests/golden/c1_fp64_traces.npz (tests/golden/make_c1_fp64.py)."""
+    from semsks import configs
+    return precision_vs_fixture(configs.c1_case(), "c1_fp64_traces.npz",
+                                "200x100 elements, 3 flat isotropic layers, plane SV p=0.06, 4000 steps, "
+                                "50 receivers, P-SV")
+
+
+def precision_aniso():
+    """The C2 model on the 200x100-element mesh (tilted-axis hexagonal upper
+    mantle, coupled P-SV+SH, 5000 steps, 50 receivers) against
+    tests/golden/aniso_fp64_traces.npz (tests/golden/make_aniso_fp64.py)."""
+    from semsks import configs
+    return precision_vs_fixture(configs.c2_model_c1_mesh(nst=5000), "aniso_fp64_traces.npz",
+                                "C2 model (crust, 4% hexagonal upper mantle, tilted axis, mantle) on 200x100 "
+                                "elements, coupled P-SV+SH, plane S 30 deg off SV, p=0.05, 5000 steps, 50 receivers")
 
 
 def kernel_time(eng, nsteps):
@@ -300,8 +321,12 @@ def main():
         if args.only is None:
             log("precision: 200x100 case, 4000 steps, FP32 GPU vs FP64 oracle")
             cfg["precision"] = precision_c1()
+            cfg["precision_aniso"] = precision_aniso()
             log("  u rel-L2 max %.2e, w rel-L2 max %.2e" % (cfg["precision"]["u"]["rel_l2_max"],
                                                          cfg["precision"]["w"]["rel_l2_max"]))
+            pa = cfg["precision_aniso"]
+            log("  anisotropic P-SV+SH case: u %.2e, w %.2e, v %.2e worst rel-L2" % (
+                pa["u"]["rel_l2_max"], pa["w"]["rel_l2_max"], pa["v"]["rel_l2_max"]))
         if r1p is not None:
             cfg.update({"c1_mesh_plain_value": r1p["throughput"], "c1_mesh_batched_value": r1b["throughput"],
                     "c1_mesh_tile_rows_plain": r1p["tile_rows"], "c1_mesh_tile_rows_batched": r1b["tile_rows"],

@@ -63,6 +63,23 @@ def c2_case(seed: int = SEED, nst: int = 20000, **kw) -> inputs.Case:
     return c
 
 
+def c2_model_c1_mesh(nst: int = 4000, seed: int = SEED, **kw) -> inputs.Case:
+    """The C2 model -- crust 30 km, an upper-mantle layer with 4% hexagonal
+    anisotropy whose axis has the C2 dip and azimuth, mantle below -- on the
+    200x100-element mesh of C1 (400 km x 200 km): coupled P-SV+SH solver,
+    plane S wave polarised 30 degrees off SV, 50 surface receivers.  Small
+    enough for the FP64 oracle to step in minutes."""
+    rng = np.random.default_rng(seed + 2)
+    dip = float(rng.uniform(0.0, 45.0))
+    az = 60.0 - 90.0
+    up = aniso.hexagonal_tilted(8.1, 4.5, 3.35, 0.04, dip, az)
+    base = dict(nex=200, eh=2.0, nez=100, layers_thk=(30.0, 120.0, 50.0), vp=(6.2, 8.1, 9.0), vs=(3.6, 4.5, 5.0),
+                rho=(2.8, 3.35, 3.6), jitter=0.0, nst=nst, nrec=50, p=0.05, f0=0.2, solver=0,
+                aniso=[None, up, None], psi_deg=30.0, sh_zero=False)
+    base.update(kw)
+    return inputs.Case(**base)
+
+
 def c3_case(nst: int = 10000, **kw) -> inputs.Case:
     """The C2 mesh with the C1-style three-layer isotropic model."""
     base = dict(nex=2000, eh=2.0, nez=500, layers_thk=(20.0, 15.0, 965.0), nst=nst, nrec=200, p=0.05, f0=0.2,

@@ -6,8 +6,9 @@ order the solver writes them (u, -w; v is zero for this solver) and in
 float32, as seis.u/seis.w hold them.
 
 bench.py reports the FP32 GPU traces' rel-L2 and peak misfit against these
-for every seismogram; tests/test_golden_c1.py pins the file against a fresh
-oracle run of its first levels.
+for every seismogram, and tests/test_gpu_c1.py holds them to 1e-4.  The
+oracle that writes them is pinned by tests/test_oracle_pins.py; running
+this script again rewrites the file bit for bit.
 
   python tests/golden/make_c1_fp64.py          (a few minutes on 8 cores)
 """
