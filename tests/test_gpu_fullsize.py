@@ -99,3 +99,26 @@ def test_mixed_degrees_in_one_engine_match_solo():
     mixed, _ = fields(probs, share=False, tile_z=4)
     for a, b in zip(mixed, solo):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_c2_full_length_stays_finite():
+    """The production configuration end to end: one instance of the
+    2000x500 mesh with the tilted-axis anisotropic upper mantle, plane SV at
+    p = 0.05, all 20000 steps.  Every trace stays finite and bounded, every
+    receiver sees the wave, and the transverse component -- zero in the
+    incident wave -- is the splitting the anisotropy produces."""
+    nst = 20000
+    print("case: c2 full mesh 2000x500, %d steps" % nst, flush=True)
+    b = configs.Built(configs.c2_case(nst=nst))
+    with semsks.Engine(max_slots=1) as e:
+        pr = b.problem(nst + 3)
+        pr.nst = nst
+        e.planewave(*b.planewave_args(nst + 3, p=0.05, psi=0.0), to_host=False)
+        s = e.admit(pr)
+        e.advance(nst - 1)
+        tr = e.traces(s)
+    assert np.isfinite(tr).all()
+    peak = np.abs(tr).max(axis=(0, 2))
+    assert 0.5 < peak[0] < 10.0 and 0.1 < peak[1] < 10.0, peak
+    assert (np.abs(tr[:, 0]).max(axis=0) > 1e-3).all()
+    assert 1e-4 * peak[0] < peak[2] < 0.1 * peak[0], peak
