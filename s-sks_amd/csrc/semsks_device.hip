@@ -2050,17 +2050,10 @@ static semsks_status build_graph(semsks_engine *e, int i)
 
 extern "C" {
 
-semsks_status semsks_engine_create(const semsks_engine_opts *opts, semsks_engine **out)
+/* device, stream and the fixed tables of a new engine; on failure the
+   caller destroys the partly built engine */
+static semsks_status engine_init(semsks_engine *e)
 {
-    *out = nullptr;
-    semsks_engine *e = new semsks_engine();
-    if (opts) e->opts = *opts;
-    if (e->opts.max_slots <= 0) e->opts.max_slots = 1;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        delete e;
-        return semsks_set_error(SEMSKS_EDEVICE, "no HIP device");
-    }
     HIPCHK(hipGetDevice(&e->device));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, e->device));
@@ -2081,7 +2074,22 @@ semsks_status semsks_engine_create(const semsks_engine_opts *opts, semsks_engine
     HIPCHK(hipMemset(e->d_pctr, 0, e->slots.size() * sizeof(int)));
     std::vector<int> z(e->slots.size(), 0);
     HIPCHK(hipMemcpy(e->d_levels, z.data(), z.size() * sizeof(int), hipMemcpyHostToDevice));
-    semsks_status st = upload_tables(e);
+    return upload_tables(e);
+}
+
+semsks_status semsks_engine_create(const semsks_engine_opts *opts, semsks_engine **out)
+{
+    if (!out) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
+    *out = nullptr;
+    semsks_engine *e = new semsks_engine();
+    if (opts) e->opts = *opts;
+    if (e->opts.max_slots <= 0) e->opts.max_slots = 1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete e;
+        return semsks_set_error(SEMSKS_EDEVICE, "no HIP device");
+    }
+    semsks_status st = engine_init(e);
     if (st != SEMSKS_OK) { semsks_engine_destroy(e); return st; }
     *out = e;
     return SEMSKS_OK;
@@ -2162,6 +2170,25 @@ static void build_schedule(int nlev, float rdt, int nstream, const int nrec[2], 
     }
 }
 
+/* device bytes a slot for p takes: its footprint, plus the two extra time
+   levels of the two-step sweep */
+static size_t admit_need(const semsks_engine *e, const semsks_problem *p, bool shared)
+{
+    return semsks_footprint(p, shared) + (e->opts.steps_per_sweep == 2 ? (size_t)ld_of(p) * nz_of(p) * 16 : 0);
+}
+
+semsks_status semsks_room(semsks_engine *e, const semsks_problem *p)
+{
+    if (!e || !p) return semsks_set_error(SEMSKS_EINVAL, "null argument");
+    bool free_slot_left = false;
+    for (auto &s : e->slots) free_slot_left = free_slot_left || !s.used;
+    if (!free_slot_left) return semsks_set_error(SEMSKS_ENOSLOT, "all %zu slots busy", e->slots.size());
+    const size_t need = admit_need(e, p, false);
+    if (e->used + need > e->budget)
+        return semsks_set_error(SEMSKS_ENOHBM, "slot needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
+    return SEMSKS_OK;
+}
+
 semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share, int *slot_out)
 {
     if (!e || !p || !slot_out) return semsks_set_error(SEMSKS_EINVAL, "null argument");
@@ -2180,8 +2207,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
             q.absorb != p->absorb)
             return semsks_set_error(SEMSKS_EINVAL, "share_slot %d has a different mesh", share);
     }
-    const size_t need = semsks_footprint(p, share >= 0) +
-                        (e->opts.steps_per_sweep == 2 ? (size_t)ld_of(p) * nz_of(p) * 16 : 0);
+    const size_t need = admit_need(e, p, share >= 0);
     if (e->used + need > e->budget)
         return semsks_set_error(SEMSKS_ENOHBM, "slot needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
     for (int i = 0; i < p->ntrace; ++i)
@@ -2553,6 +2579,7 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
 
 semsks_status semsks_last_timing(semsks_engine *e, double out[2])
 {
+    if (!e || !out) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
     out[0] = e->last_ms[0];
     out[1] = e->last_ms[1];
     return SEMSKS_OK;
@@ -2562,6 +2589,7 @@ int semsks_last_timed_steps(semsks_engine *e) { return e ? e->timed_steps : 0; }
 
 semsks_status semsks_sync(semsks_engine *e)
 {
+    if (!e) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
     HIPCHK(hipStreamSynchronize(e->stream));
     return SEMSKS_OK;
 }
@@ -2623,8 +2651,13 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
     const Slot &s = e->slots[k];
     const size_t nodes = (size_t)s.ld * s.nz;
+    struct Scratch {           /* freed on every return (hipFree waits for the device) */
+        void *p[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Scratch() { for (void *q : p) if (q) hipFree(q); }
+    } sc;
     float *buf = nullptr;
     HIPCHK(hipMalloc(&buf, nodes * 6 * sizeof(float) + 256));
+    sc.p[0] = buf;
     HIPCHK(hipMemsetAsync(buf, 0, nodes * 6 * sizeof(float), e->stream));
     InstDesc d = s.d;
     std::vector<float> aos(nodes * 3, 0.f);
@@ -2643,8 +2676,11 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     build_tiles(s.share >= 0 ? e->slots[s.share] : s, 0, tile_rows(e, s.meta.nez), false, t);
     const int cnt = (int)t.size();
     HIPCHK(hipMalloc(&dd, sizeof d));
+    sc.p[1] = dd;
     HIPCHK(hipMalloc(&tl, t.size() * sizeof(int4)));
+    sc.p[2] = tl;
     HIPCHK(hipMalloc(&nt, sizeof cnt));
+    sc.p[3] = nt;
     HIPCHK(hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(tl, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(nt, &cnt, sizeof cnt, hipMemcpyHostToDevice, e->stream));
@@ -2654,7 +2690,6 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
         HIPCHK(hipMemcpy2DAsync(f + (size_t)c * s.nx * s.nz, s.nx * 4, d.fout[c], s.ld * 4, s.nx * 4, s.nz,
                                 hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    hipFree(buf); hipFree(dd); hipFree(tl); hipFree(nt);
     return SEMSKS_OK;
 }
 

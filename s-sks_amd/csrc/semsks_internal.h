@@ -8,6 +8,10 @@ extern "C" {
 /* records an error text for semsks_last_error() and returns st */
 semsks_status semsks_set_error(semsks_status st, const char *fmt, ...);
 
+/* whether semsks_admit(e, p, -1, ...) would find a free slot and the HBM
+   for p: SEMSKS_OK, SEMSKS_ENOSLOT or SEMSKS_ENOHBM, with no side effect */
+semsks_status semsks_room(semsks_engine *e, const semsks_problem *p);
+
 /* getpar semantics (semsyn/src/getpar/src/getpar.c) for the library's
    readers: open = setpar, semsks_par_* = getpar (1 if the key is present),
    semsks_mstpar_* = mstpar, close = endpar (EPAR for a missing mstpar key,

@@ -37,7 +37,11 @@ static semsks_status try_admit(semsks_engine *e, semsks_request *r, int *slot)
 {
     const semsks_problem *p = r->problem;
     if (r->use_planewave) {
-        semsks_status st = semsks_planewave(e, p->nex, p->nez, p->hpn, p->neh, p->eh, p->nt, p->dp, r->planewave[0],
+        /* the records are generated only for a request that fits now, not
+           again on every round while it waits */
+        semsks_status st = semsks_room(e, p);
+        if (st != SEMSKS_OK) return st;
+        st = semsks_planewave(e, p->nex, p->nez, p->hpn, p->neh, p->eh, p->nt, p->dp, r->planewave[0],
                                             r->planewave[1], r->planewave[2], r->planewave[3], NULL, NULL);
         if (st != SEMSKS_OK) return st;
     }
