@@ -230,6 +230,41 @@ def test_serving_results_equal_alone_and_in_the_mix(tmp_path, monkeypatch):
             assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, c, l2, pk)
 
 
+def test_serving_waits_for_memory_and_rejects_what_never_fits():
+    """Admission by HBM footprint: with a budget for two of the small
+    requests, the third waits until one retires; a request larger than the
+    whole budget comes back as SEMSKS_ENOHBM once the engine is idle,
+    without holding up the request behind it.  Every request generates its
+    boundary records on the device, and each served result equals its run
+    alone, bit for bit."""
+    from semsks import configs
+    print("case: serving under a 2-request HBM budget, plane-wave records on the device", flush=True)
+    cases = [inputs.small_case(nex=40, nst=400),
+             inputs.small_case(nex=44, nst=320, p=0.05, psi_deg=35.0, sh_zero=False),
+             inputs.small_case(nex=42, nst=450, p=0.07, solver=1),
+             inputs.small_case(nex=600, nst=50),                               # larger than the budget
+             inputs.small_case(nex=40, nst=360, p=0.045, psi_deg=60.0, sh_zero=False)]
+    built = [configs.Built(c) for c in cases]
+    probs = [b.problem(b.case.nst + 3) for b in built]
+    pws = [b.planewave_args(p.nt)[7:11] for b, p in zip(built, probs)]
+    small_fp = [semsks.footprint(p) for k, p in enumerate(probs) if k != 3]
+    budget = 2 * max(small_fp) + min(small_fp) // 2
+    assert semsks.footprint(probs[3]) > budget
+    arrivals = [0.0, 0.0, 0.0, 0.001, 0.002]
+    with semsks.Engine(max_slots=8, hbm_budget=budget) as e:
+        res, stats = e.serve(probs, arrivals, pws, steps_per_round=16)
+    assert res[3]["status"] == 3 and res[3]["traces"] is not None
+    assert stats["completed"] == 4 and stats["max_active"] == 2
+    assert res[2]["admit"] > res[2]["arrival"]                 # waited for memory
+    assert res[4]["admit"] >= res[3]["done"]                   # admitted once the large one was turned away
+    for k in (0, 1, 2, 4):
+        with semsks.Engine(max_slots=1) as e:
+            (solo,), _ = e.serve([probs[k]], [0.0], [pws[k]], steps_per_round=16)
+        assert res[k]["status"] == 0 and solo["status"] == 0
+        assert np.abs(solo["traces"]).max() > 0
+        assert np.array_equal(res[k]["traces"].view(np.uint32), solo["traces"].view(np.uint32)), k
+
+
 @pytest.mark.parametrize("atten", [1, 4])
 def test_sponge_strips_vs_fp64(small, atten):
     """atten_wave2's exponential taper on the edge strips (bottom only, and
