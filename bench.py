@@ -243,6 +243,9 @@ def main():
                     help="2: two time steps per sweep of the mesh (step2_kernel) where possible")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one device copy of this many MiB (PMC byte-counter calibration)")
+    ap.add_argument("--stream-ceiling", type=int, default=0,
+                    help="only time a device copy and a triad a = b + 2c over arrays of this many MiB "
+                         "(the practical HBM rate for 1:1 and 2:1 read:write traffic)")
     args = ap.parse_args()
     global STEPS_PER_SWEEP
     STEPS_PER_SWEEP = args.steps_per_sweep
@@ -255,6 +258,26 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+        return
+    if args.stream_ceiling:
+        import torch
+        n = args.stream_ceiling * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b, c = torch.full_like(a, 2.0), torch.full_like(a, 3.0)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        res = {}
+        for name, fn, nbytes in (("copy", lambda: b.copy_(a), 8 * n),
+                                 ("triad", lambda: torch.add(b, c, alpha=2.0, out=a), 12 * n)):
+            best = float("inf")
+            for r in range(8):
+                ev[0].record()
+                fn()
+                ev[1].record()
+                torch.cuda.synchronize()
+                if r >= 2:
+                    best = min(best, ev[0].elapsed_time(ev[1]))
+            res[name] = {"bytes": nbytes, "best_ms": best, "GB_per_s": nbytes / best * 1e-6}
+        print(json.dumps(res))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only (there is no multi-GPU path); got --gpus %d" % args.gpus,

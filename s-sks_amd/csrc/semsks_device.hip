@@ -68,6 +68,22 @@
 #include "semsks_internal.h"
 #include "semsks_kernels.h"
 
+/* node offset in a plane: column blocks of bw node columns, each stored row
+   after row (one block holds the owned columns of one step-kernel tile) */
+__host__ __device__ static inline long node_col(int bw, int nz, int ix)
+{
+    const int blk = ix / bw;
+    return (long)blk * bw * nz + (ix - blk * bw);
+}
+__host__ __device__ static inline long node_at(int bw, int nz, int ix, int iz)
+{
+    return node_col(bw, nz, ix) + (long)iz * bw;
+}
+__host__ __device__ static inline long node_at(const InstDesc &D, int ix, int iz)
+{
+    return node_at(D.bw, D.nz, ix, iz);
+}
+
 #define HIPCHK(x)                                                                         \
     do {                                                                                  \
         hipError_t e_ = (x);                                                              \
@@ -176,7 +192,7 @@ __global__ void mass_kernel(InstDesc D, const double *w64, double je, float sdt,
         if (k == 0 && iz > 0 && iz < D.nz - 1) cz = 2;
         if (ix == D.nx - 1) m = n;
         if (iz == D.nz - 1) k = n;
-        double rho = (double)D.mat[(long)D.mat_id[(long)iz * D.ld + ix] * SEMSKS_MAT_STRIDE];
+        double rho = (double)D.mat[(long)D.mat_id[node_at(D, ix, iz)] * SEMSKS_MAT_STRIDE];
         double t = w64[m] * w64[k] * je * rho, s = 0.0;
         for (int c = 0; c < cx * cz; ++c) s = s + t;
         if (mass) mass[q] = s;
@@ -212,7 +228,7 @@ __global__ void expand_mdt2_kernel(InstDesc D, float *out)
         const int iz = (int)(q / D.nx), ix = (int)(q % D.nx);
         const int cx = (ix == 0 || ix == D.nx - 1) ? n : ix % n;
         const int cz = (iz == 0 || iz == D.nz - 1) ? n : iz % n;
-        const int mid = D.mat_id[(long)iz * D.ld + ix];
+        const int mid = D.mat_id[node_at(D, ix, iz)];
         out[q] = D.mclass[(long)mid * NC * NC + cz * NC + cx];
     }
 }
@@ -239,7 +255,7 @@ __global__ void absorb_kernel(InstDesc D, const double *w64, double jb, float sd
         else if (e < nx + 2 * (nz - 1)) { ix = nx - 1; iz = e - nx - (nz - 1) + 1; }
         else { ix = e - nx - 2 * (nz - 1) + 1; iz = nz - 1; }
         if (ix >= nx) continue;
-        long node = (long)iz * D.ld + ix;
+        long node = node_at(D, ix, iz);
         double A[3] = {0, 0, 0};
         if (absorb) {
             for (int k = 0; k < 3; ++k) {
@@ -319,7 +335,7 @@ __global__ void series_kernel(const float *rec, int width, int nrec, const Sched
    and column from g(0); u(-dt) = 0 (the cleared buffer of level -1) */
 __global__ void init_state_kernel(InstDesc D)
 {
-    const int nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz, ld = D.ld;
+    const int nbl = D.neh * D.hpn, nx = D.nx, nz = D.nz;
     const float *g = D.G, *gv = D.Gv;
     const bool sh = D.solver == SEMSKS_SOLVER_PSVSH;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nx + nz; i += gridDim.x * blockDim.x) {
@@ -329,12 +345,12 @@ __global__ void init_state_kernel(InstDesc D)
             int ix = i;
             if (ix < nbl) continue;
             if (ix == nbl) continue;                 /* the left pass writes the corner */
-            q = (long)nbl * ld + ix;
+            q = node_at(D, ix, nbl);
             u = g[ix]; w = -g[nx + ix]; if (sh) v = gv[ix];
         } else {
             int iz = i - nx;
             if (iz < nbl) continue;
-            q = (long)iz * ld + nbl;
+            q = node_at(D, nbl, iz);
             u = g[2 * nx + iz]; w = -g[2 * nx + nz + iz]; if (sh) v = gv[nx + iz];
         }
         D.u[0][3 * q] = u; D.u[0][3 * q + 1] = v; D.u[0][3 * q + 2] = w;
@@ -456,10 +472,10 @@ __device__ __forceinline__ bool is_inject_node(int ix, int iz, int nbl)
    corner included) and then the left column ix = nbl (iz > nbl). */
 __device__ __forceinline__ void inject_node(const InstDesc &D, int it, int i, int nb)
 {
-    const int L = D.nlev, p = it % L, nbl = D.neh * D.hpn, ld = D.ld;
+    const int L = D.nlev, p = it % L, nbl = D.neh * D.hpn;
     const int ix = i < nb ? nbl + i : nbl;
     const int iz = i < nb ? nbl : nbl + 1 + (i - nb);
-    const long q = (long)iz * ld + ix;
+    const long q = node_at(D, ix, iz);
     const bool mp = D.solver == SEMSKS_SOLVER_PSV_MP;
     float *U = D.u[p];
     const float *Up = D.u[(p + L - 1) % L];
@@ -497,7 +513,7 @@ __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
     if (!D.active || D.atten <= 0) return;
     const int it = levels[blockIdx.y];
     if (it >= D.nst - 1) return;
-    const int pn = (it + 1) % D.nlev, nh = D.nh, nx = D.nx, nz = D.nz, ld = D.ld, at = D.atten;
+    const int pn = (it + 1) % D.nlev, nh = D.nh, nx = D.nx, nz = D.nz, at = D.atten;
     const long nb = (long)nh * nx, ns = (long)(nz - nh) * nh, ntop = (long)nh * (nx - 2 * nh);
     const long total = nb + (at > 1 ? ns : 0) + (at > 2 ? ns : 0) + (at > 3 ? ntop : 0);
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
@@ -518,7 +534,7 @@ __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
             iz = nz - nh + (int)(j / (nx - 2 * nh)); ix = nh + (int)(j % (nx - 2 * nh));
             f = D.taper[iz - nz + nh];
         }
-        const long q = (long)iz * ld + ix;
+        const long q = node_at(D, ix, iz);
         float *U1 = D.u[pn] + 3 * q;
         for (int k = 0; k < 3; ++k) U1[k] = f * U1[k];
         D.comp[pn][q] &= ~C_OWN;          /* the previous level's fields stay */
@@ -855,7 +871,8 @@ __device__ __forceinline__ void node_stress(const float *Cu, const SEMSKS_GMEM f
 template <int N, bool UNIFORM, bool ORTHO>
 __device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)[N + 1][3], int b,
                                           const RowCoef<N> &rc, float *Ub, float *Qb, const float *Cu,
-                                          const SEMSKS_GMEM float *mat, const SEMSKS_GMEM uint16_t *idrow, bool wr)
+                                          const SEMSKS_GMEM float *mat, const SEMSKS_GMEM uint16_t *idrow,
+                                          const SEMSKS_GMEM uint16_t *idn, bool wr)
 {
     constexpr int NP = N + 1;
     constexpr int CB = RowLayout<N>::CB;
@@ -886,7 +903,8 @@ __device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)
             gz2 += rc.dz[t] * col[2 * NP + t];
         }
         float s0, s2, s3, s4, s5;
-        const SEMSKS_GMEM float *crec = UNIFORM ? nullptr : mat + (long)idrow[a] * SEMSKS_MAT_STRIDE;
+        const SEMSKS_GMEM float *crec =
+            UNIFORM ? nullptr : mat + (long)(a < N ? idrow[a] : idn[0]) * SEMSKS_MAT_STRIDE;
         node_stress<UNIFORM, ORTHO>(Cu, crec, gx0, gz2, gz1, gx2 + gz0, gx1, s0, s2, s3, s4, s5);
         const float wa = opW<N>(a);
         const float p0 = rc.wb * s0, p1 = rc.wb * s5, p2 = rc.wb * s4;   /* row traction    T_i0 */
@@ -979,7 +997,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
 {
     using RL = RowLayout<N>;
     constexpr int NP = RL::NP;
-    const int nex = D.nex, nez = D.nez, ld = D.ld;
+    const int nex = D.nex, nez = D.nez, bw = D.bw;
     const int e = lane / NP, b = lane - e * NP;
     const bool act = lane < RL::NL;
     const int ex = x0 - 1 + e;
@@ -995,6 +1013,9 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
     const int exl = valid ? ex : 0;
     const int ex_first = x0 > 0 ? x0 - 1 : 0;
     const RowCoef<N> rc = row_coef<N>(b);
+    /* storage columns of the lane's nodes 0..N-1 and of node N, the first
+       node of the next element, which may open the next block */
+    const uint32_t colb = (uint32_t)node_col(bw, D.nz, exl * N), colbn = (uint32_t)node_col(bw, D.nz, exl * N + N);
 
     const SEMSKS_GMEM float *U = gp(D.u[p]);
     const int pp = (p + D.nlev - 1) % D.nlev, pn = (p + 1) % D.nlev;   /* buffers of levels t-dt, t+dt */
@@ -1007,11 +1028,11 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
        moves it across lanes) */
     auto fetch = [&](int ez, RowFetch<N> &r, bool share) {
         const int iz = ez * N + b;
-        const long rowoff = (long)iz * ld + (long)exl * N;
+        const long rowoff = (long)(colb + (uint32_t)(iz * bw));
         const bool mem = !(share && b == 0);
         row_load<N>(U + 3 * rowoff, valid && mem, r.u);
         r.un[0] = r.un[1] = r.un[2] = 0.f;
-        if (direct && mem) xyz_load(U, rowoff + N, r.un);
+        if (direct && mem) xyz_load(U, (long)(colbn + (uint32_t)(iz * bw)), r.un);
         r.rid = 0;
         if constexpr (!MIXED) r.rid = (int)cp(D.emat)[(long)ez * nex + ex_first];
         if constexpr (N == 4 && !FORCE_ONLY) {
@@ -1073,8 +1094,8 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
         const int iz = ez * N + b;
         const bool top = ez == nez - 1;
         const bool own = own_x && ez >= z0 && (b < N || top);
-        const long rowoff = (long)iz * ld + (long)exl * N;
-        const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
+        const long rowoff = (long)(colb + (uint32_t)(iz * bw)), offn = (long)(colbn + (uint32_t)(iz * bw));
+        const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff, *idn = gp(D.mat_id) + offn;
         const int rid = cur.rid;
         /* this lane's row of u(t); column N from the right neighbour lane */
         float u[NP][3];
@@ -1095,9 +1116,9 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
             float C[15];
 #pragma unroll
             for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
-            row_force<N, true, ORTHO>(u, f, b, rc, Ub, Qb, C, gp(D.mat), idrow, act);
+            row_force<N, true, ORTHO>(u, f, b, rc, Ub, Qb, C, gp(D.mat), idrow, idn, act);
         } else {
-            row_force<N, false, ORTHO>(u, f, b, rc, Ub, Qb, nullptr, gp(D.mat), idrow, act);
+            row_force<N, false, ORTHO>(u, f, b, rc, Ub, Qb, nullptr, gp(D.mat), idrow, idn, act);
         }
         if (!valid) {
 #pragma unroll
@@ -1146,7 +1167,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
         }
         if (own) {
             const int ixb = ex * N;
-            const long q0 = (long)iz * ld + ixb;
+            const long q0 = rowoff;
             const int a1 = last_x ? NP : N;
             const bool vec = N == 4 && b < N && iz != 0 && ex != 0;
             if constexpr (FORCE_ONLY) {
@@ -1154,7 +1175,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
                 for (int a = 0; a < NP; ++a)
                     if (a < a1)
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) gp(D.fout[k])[q0 + a] = f[a][k];
+                        for (int k = 0; k < 3; ++k) gp(D.fout[k])[a < N ? q0 + a : offn] = f[a][k];
             } else if (N == 4 && vec) {
                 float nv[4][3];
                 uint32_t nw[4];
@@ -1171,7 +1192,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
                 if (last_x) {
                     float fa[3] = {f[N][0], f[N][1], f[N][2]};
                     float ua[3] = {u[N][0], u[N][1], u[N][2]};
-                    update_one<N>(D, p, q0 + N, ixb + N, iz, MIXED ? (int)idrow[N] : rid, fa, ua);
+                    update_one<N>(D, p, offn, ixb + N, iz, MIXED ? (int)idn[0] : rid, fa, ua);
                 }
             } else {
 #pragma unroll
@@ -1179,7 +1200,8 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
                     if (a >= a1) continue;
                     float fa[3] = {f[a][0], f[a][1], f[a][2]};
                     float ua[3] = {u[a][0], u[a][1], u[a][2]};
-                    update_one<N>(D, p, q0 + a, ixb + a, iz, MIXED ? (int)idrow[a] : rid, fa, ua);
+                    update_one<N>(D, p, a < N ? q0 + a : offn, ixb + a, iz,
+                                  MIXED ? (int)(a < N ? idrow[a] : idn[0]) : rid, fa, ua);
                 }
             }
         }
@@ -1296,7 +1318,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     constexpr int N = 4;
     using RL = RowLayout<N>;
     constexpr int NP = RL::NP;
-    const int nex = D.nex, nez = D.nez, ld = D.ld, nx = D.nx, nz = D.nz;
+    const int nex = D.nex, nez = D.nez, bw = D.bw, nx = D.nx, nz = D.nz;
     const int e = lane / NP, b = lane - e * NP;
     const bool act = lane < RL::NL;
     const int ex = x0 - 2 + e;
@@ -1313,6 +1335,10 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     const int exl = valid ? ex : 0;
     const int ex_first = x0 >= 2 ? x0 - 2 : 0;
     const RowCoef<N> rc = row_coef<N>(b);
+    /* storage columns of the lane's nodes 0..N-1 and of node N (the tiles of
+       a two-step sweep are narrower than a storage block, so elements of one
+       tile may lie in two blocks; an element never does) */
+    const uint32_t colb = (uint32_t)node_col(bw, nz, exl * N), colbn = (uint32_t)node_col(bw, nz, exl * N + N);
     const int nbl = D.neh * N;
     const bool inj = two && injects(D, it + 1);
 
@@ -1331,7 +1357,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
 
     auto fetch = [&](int ez, RowFetch<N> &r) {
         const int iz = ez * N + b;
-        const long rowoff = (long)iz * ld + (long)exl * N;
+        const long rowoff = (long)(colb + (uint32_t)(iz * bw));
         row_load<N>(U + 3 * rowoff, valid, r.u);
         uint4 w4 = make_uint4(0u, 0u, 0u, 0u);
         if (valid) w4 = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(W + rowoff);
@@ -1339,8 +1365,8 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
         r.un[0] = r.un[1] = r.un[2] = 0.f;
         r.wn = 0u;
         if (direct) {
-            xyz_load(U, rowoff + N, r.un);
-            r.wn = W[rowoff + N];
+            xyz_load(U, (long)(colbn + (uint32_t)(iz * bw)), r.un);
+            r.wn = W[(long)(colbn + (uint32_t)(iz * bw))];
         }
         r.rid = 0;
         if constexpr (!MIXED) r.rid = (int)cp(D.emat)[(long)ez * nex + ex_first];
@@ -1348,7 +1374,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     /* u(t-dt) and dt^2/M of the row, loaded at the top of its first step */
     auto fetch_prev = [&](int ez, RowFetch<N> &r) {
         const int iz = ez * N + b;
-        const long rowoff = (long)iz * ld + (long)exl * N;
+        const long rowoff = (long)(colb + (uint32_t)(iz * bw));
         const bool on = valid && (b < N || ez == nez - 1);
         r.m[0] = r.m[1] = r.m[2] = r.m[3] = 0.f;
         row_load<4>(UP + 3 * rowoff, on, r.pv);
@@ -1424,16 +1450,17 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
     };
 
     auto row_material = [&](int ez, int rid, float (&f)[NP][3], const float (&u)[NP][3]) {
-        const long rowoff = (long)(ez * N + b) * ld + (long)exl * N;
+        const long rowoff = (long)(colb + (uint32_t)((ez * N + b) * bw));
         const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
+        const SEMSKS_GMEM uint16_t *idn = gp(D.mat_id) + (long)(colbn + (uint32_t)((ez * N + b) * bw));
         if constexpr (!MIXED) {
             const SEMSKS_CMEM float *r = cp(D.mat) + (long)rid * SEMSKS_MAT_STRIDE;
             float C[15];
 #pragma unroll
             for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
-            row_force<N, true, ORTHO>(u, f, b, rc, Ub, Qb, C, gp(D.mat), idrow, act);
+            row_force<N, true, ORTHO>(u, f, b, rc, Ub, Qb, C, gp(D.mat), idrow, idn, act);
         } else {
-            row_force<N, false, ORTHO>(u, f, b, rc, Ub, Qb, nullptr, gp(D.mat), idrow, act);
+            row_force<N, false, ORTHO>(u, f, b, rc, Ub, Qb, nullptr, gp(D.mat), idrow, idn, act);
         }
     };
 
@@ -1486,7 +1513,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
                 }
                 wu[N] = cur.wn;
                 pv[N][0] = pv[N][1] = pv[N][2] = 0.f;
-                if (last_x) xyz_load(UP, (long)iz * ld + nx - 1, pv[N]);
+                if (last_x) xyz_load(UP, node_at(D, nx - 1, iz), pv[N]);
                 row_next(iz, f, u, wu, pv, cur.m, v1, w1);
                 if (inj && (iz == nbl || (ex * N <= nbl && nbl <= ex * N + N))) {
 #pragma unroll
@@ -1500,7 +1527,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
                     }
                 }
                 if (own_x && ez >= z0 && ez < z0 + rows) {
-                    const long q = (long)iz * ld + (long)ex * N;
+                    const long q = (long)(colb + (uint32_t)(iz * bw)), qn = (long)(colbn + (uint32_t)(iz * bw));
                     float o[4][3];
 #pragma unroll
                     for (int a = 0; a < N; ++a)
@@ -1509,8 +1536,8 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
                     row_store4(U1 + 3 * q, o);
                     *reinterpret_cast<SEMSKS_GMEM uint4 *>(W1 + q) = make_uint4(w1[0], w1[1], w1[2], w1[3]);
                     if (last_x) {
-                        xyz_store(U1, q + N, v1[N]);
-                        W1[q + N] = w1[N];
+                        xyz_store(U1, qn, v1[N]);
+                        W1[qn] = w1[N];
                     }
                 }
             }
@@ -1554,7 +1581,7 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
                 }
                 pv[N][0] = prv.un[0]; pv[N][1] = prv.un[1]; pv[N][2] = prv.un[2];
                 row_next(iz, f, v1p, w1p, pv, prv.m, nu, nw);
-                const long q = (long)iz * ld + (long)ex * N;
+                const long q = (long)(colb + (uint32_t)(iz * bw)), qn = (long)(colbn + (uint32_t)(iz * bw));
                 float o[4][3];
 #pragma unroll
                 for (int a = 0; a < N; ++a)
@@ -1563,8 +1590,8 @@ __device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0
                 row_store4(U2 + 3 * q, o);
                 *reinterpret_cast<SEMSKS_GMEM uint4 *>(W2 + q) = make_uint4(nw[0], nw[1], nw[2], nw[3]);
                 if (last_x) {
-                    xyz_store(U2, q + N, nu[N]);
-                    W2[q + N] = nw[N];
+                    xyz_store(U2, qn, nu[N]);
+                    W2[qn] = nw[N];
                 }
             }
         }
@@ -1665,7 +1692,8 @@ struct Slot {
     int refs = 0;             /* slots sharing this one's material */
     semsks_problem meta{};
     InstDesc d{};
-    int nx = 0, nz = 0, ld = 0, nt_sem = 0, nlev = 0;
+    int nx = 0, nz = 0, bw = 0, nt_sem = 0, nlev = 0;
+    size_t plane = 0;                 /* nodes per level plane (storage blocks, padded) */
     size_t bytes = 0;
     std::vector<void *> allocs;       /* owned device allocations */
     std::vector<uint16_t> emat;       /* host copy of the element-uniform record ids (owner) */
@@ -2121,12 +2149,20 @@ void semsks_engine_destroy(semsks_engine *e)
 
 static int nx_of(const semsks_problem *p) { return p->nex * p->hpn + 1; }
 static int nz_of(const semsks_problem *p) { return p->nez * p->hpn + 1; }
-static int ld_of(const semsks_problem *p) { return ((nx_of(p) + 3 + 63) / 64) * 64; }
+/* storage block width: the owned node columns of one step-kernel tile */
+static int bw_of(const semsks_problem *p) { return xchunk_of(p->hpn) * p->hpn; }
+/* nodes per level plane: whole blocks, rounded up to 64 so that every plane
+   of a slot starts on a 256-byte boundary */
+static size_t plane_of(const semsks_problem *p)
+{
+    const size_t bw = bw_of(p), nblk = (nx_of(p) + bw - 1) / bw;
+    return (nblk * bw * nz_of(p) + 63) / 64 * 64;
+}
 static int nt_sem_of(const semsks_problem *p) { return (int)((float)((p->nt - 3) * p->dp / p->sdt)); }
 
 size_t semsks_footprint(const semsks_problem *p, int shared)
 {
-    const size_t nodes = (size_t)ld_of(p) * nz_of(p);
+    const size_t nodes = plane_of(p);
     size_t b = nodes * 4 * (3 * 4);                 /* three levels: xyz plane + compensation words */
     if (!shared) b += nodes * 2 + (size_t)p->nex * p->nez * 2 +
                       (size_t)p->n_mat * (64 + 4 * (p->hpn + 1) * (p->hpn + 1)) +
@@ -2174,7 +2210,7 @@ static void build_schedule(int nlev, float rdt, int nstream, const int nrec[2], 
    levels of the two-step sweep */
 static size_t admit_need(const semsks_engine *e, const semsks_problem *p, bool shared)
 {
-    return semsks_footprint(p, shared) + (e->opts.steps_per_sweep == 2 ? (size_t)ld_of(p) * nz_of(p) * 16 : 0);
+    return semsks_footprint(p, shared) + (e->opts.steps_per_sweep == 2 ? plane_of(p) * 16 : 0);
 }
 
 semsks_status semsks_room(semsks_engine *e, const semsks_problem *p)
@@ -2196,6 +2232,8 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         !(p->eh > 0) || p->n_mat < 1 || !p->mat_table || !p->mat_id || p->ntrace < 0 || p->nt < 0 ||
         (p->solver < 0 || p->solver > 2))
         return semsks_set_error(SEMSKS_EINVAL, "invalid problem description");
+    if (plane_of(p) >= ((size_t)1 << 31))   /* node offsets are 32-bit in the kernels */
+        return semsks_set_error(SEMSKS_EINVAL, "mesh of %zu nodes per plane is too large", plane_of(p));
     int k = -1;
     for (size_t i = 0; i < e->slots.size(); ++i) if (!e->slots[i].used) { k = (int)i; break; }
     if (k < 0) return semsks_set_error(SEMSKS_ENOSLOT, "all %zu slots busy", e->slots.size());
@@ -2219,7 +2257,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     Slot &s = e->slots[k];
     s = Slot();
     s.meta = *p;
-    s.nx = nx_of(p); s.nz = nz_of(p); s.ld = ld_of(p);
+    s.nx = nx_of(p); s.nz = nz_of(p); s.bw = bw_of(p); s.plane = plane_of(p);
     s.nt_sem = nt_sem_of(p);
     s.nlev = std::max(1, std::min(s.nt_sem, p->nst));
     s.share = share;
@@ -2230,14 +2268,14 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
             s.two_ok = false;
     InstDesc &d = s.d;
     memset(&d, 0, sizeof d);
-    d.nex = p->nex; d.nez = p->nez; d.nx = s.nx; d.nz = s.nz; d.ld = s.ld;
+    d.nex = p->nex; d.nez = p->nez; d.nx = s.nx; d.nz = s.nz; d.bw = s.bw;
     d.hpn = p->hpn; d.neh = p->neh; d.nst = p->nst; d.nt_sem = s.nt_sem; d.solver = p->solver;
     d.kahan = (p->flags & SEMSKS_F_NO_KAHAN) ? 0 : 1;
     d.ortho = 1;     /* every record orthotropic in the mesh axes: the short stress form */
     for (int i = 0; i < p->n_mat; ++i)
         if (!semsks_ortho_record(p->mat_table + (size_t)i * SEMSKS_MAT_STRIDE + 1)) d.ortho = 0;
     d.ntrace = p->ntrace;
-    const size_t nodes = (size_t)s.ld * s.nz;
+    const size_t nodes = s.plane;
     const int mt = 2 * (s.nx + s.nz), sht = s.nx + s.nz;
     semsks_status st = SEMSKS_OK;
 #define ACHK(x) do { if ((x) != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "device allocation failed"); goto fail; } } while (0)
@@ -2288,8 +2326,14 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         const int ne = semsks_n_edge(s.nx, s.nz, p->hpn);
         ACHK(dalloc(s, &ea, (size_t)ne * 3));
         ACHK(dalloc(s, &eb, (size_t)ne * 3));
-        ACHK(hipMemsetAsync(mid, 0, nodes * 2, e->stream));
-        ACHK(hipMemcpy2DAsync(mid, s.ld * 2, p->mat_id, s.nx * 2, s.nx * 2, s.nz, hipMemcpyHostToDevice, e->stream));
+        {   /* record ids in the block order of the level planes */
+            std::vector<uint16_t> ids(nodes, 0);
+            for (int iz = 0; iz < s.nz; ++iz)
+                for (int ix = 0; ix < s.nx; ++ix)
+                    ids[node_at(s.bw, s.nz, ix, iz)] = p->mat_id[(size_t)iz * s.nx + ix];
+            ACHK(hipMemcpyAsync(mid, ids.data(), nodes * 2, hipMemcpyHostToDevice, e->stream));
+            ACHK(hipStreamSynchronize(e->stream));
+        }
         ACHK(hipMemcpyAsync(mat, p->mat_table, (size_t)p->n_mat * SEMSKS_MAT_STRIDE * 4, hipMemcpyHostToDevice, e->stream));
         {   /* element-uniform record ids */
             std::vector<uint16_t> em((size_t)p->nex * p->nez);
@@ -2386,7 +2430,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         int *rec = nullptr;
         float *tr = nullptr;
         std::vector<int> off(std::max(p->ntrace, 1), 0);
-        for (int i = 0; i < p->ntrace; ++i) off[i] = p->rec_iz[i] * s.ld + p->rec_ix[i];
+        for (int i = 0; i < p->ntrace; ++i) off[i] = (int)node_at(s.bw, s.nz, p->rec_ix[i], p->rec_iz[i]);
         ACHK(dalloc(s, &rec, off.size()));
         ACHK(dalloc(s, &tr, (size_t)p->nst * 3 * std::max(p->ntrace, 1)));
         ACHK(hipMemcpyAsync(rec, off.data(), off.size() * 4, hipMemcpyHostToDevice, e->stream));
@@ -2636,13 +2680,13 @@ semsks_status semsks_get_field(semsks_engine *e, int k, float *out)
     if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
     const Slot &s = e->slots[k];
     const int p = e->h_levels[k] % s.d.nlev;
-    std::vector<float> aos((size_t)s.ld * s.nz * 3);
+    std::vector<float> aos(s.plane * 3);
     HIPCHK(hipMemcpyAsync(aos.data(), s.d.u[p], aos.size() * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     for (int c = 0; c < 3; ++c)
         for (int iz = 0; iz < s.nz; ++iz)
             for (int ix = 0; ix < s.nx; ++ix)
-                out[((size_t)c * s.nz + iz) * s.nx + ix] = aos[((size_t)iz * s.ld + ix) * 3 + c];
+                out[((size_t)c * s.nz + iz) * s.nx + ix] = aos[node_at(s.bw, s.nz, ix, iz) * 3 + c];
     return SEMSKS_OK;
 }
 
@@ -2650,7 +2694,7 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
 {
     if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
     const Slot &s = e->slots[k];
-    const size_t nodes = (size_t)s.ld * s.nz;
+    const size_t nodes = s.plane;
     struct Scratch {           /* freed on every return (hipFree waits for the device) */
         void *p[4] = {nullptr, nullptr, nullptr, nullptr};
         ~Scratch() { for (void *q : p) if (q) hipFree(q); }
@@ -2664,7 +2708,7 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     for (int c = 0; c < 3; ++c)
         for (int iz = 0; iz < s.nz; ++iz)
             for (int ix = 0; ix < s.nx; ++ix)
-                aos[((size_t)iz * s.ld + ix) * 3 + c] = u[((size_t)c * s.nz + iz) * s.nx + ix];
+                aos[node_at(s.bw, s.nz, ix, iz) * 3 + c] = u[((size_t)c * s.nz + iz) * s.nx + ix];
     d.u[0] = buf;
     for (int c = 0; c < 3; ++c) d.fout[c] = buf + (3 + c) * nodes;
     HIPCHK(hipMemcpyAsync(d.u[0], aos.data(), aos.size() * 4, hipMemcpyHostToDevice, e->stream));
@@ -2686,10 +2730,13 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     HIPCHK(hipMemcpyAsync(nt, &cnt, sizeof cnt, hipMemcpyHostToDevice, e->stream));
     launch_tiles<true>(e, s.meta.hpn, dd, e->d_levels, tl, nt);
     HIPCHK(hipGetLastError());
-    for (int c = 0; c < 3; ++c)
-        HIPCHK(hipMemcpy2DAsync(f + (size_t)c * s.nx * s.nz, s.nx * 4, d.fout[c], s.ld * 4, s.nx * 4, s.nz,
-                                hipMemcpyDeviceToHost, e->stream));
+    std::vector<float> fb(nodes * 3);
+    HIPCHK(hipMemcpyAsync(fb.data(), d.fout[0], fb.size() * 4, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    for (int c = 0; c < 3; ++c)
+        for (int iz = 0; iz < s.nz; ++iz)
+            for (int ix = 0; ix < s.nx; ++ix)
+                f[((size_t)c * s.nz + iz) * s.nx + ix] = fb[c * nodes + node_at(s.bw, s.nz, ix, iz)];
     return SEMSKS_OK;
 }
 
@@ -2705,7 +2752,7 @@ semsks_status semsks_get_setup(semsks_engine *e, int k, double *mass, float *mdt
     HIPCHK(hipMalloc(&w64, SEMSKS_MAXNP * sizeof(double)));
     HIPCHK(hipMalloc(&dm, N * sizeof(double)));
     HIPCHK(hipMalloc(&dab, N * 3 * sizeof(double)));
-    HIPCHK(hipMalloc(&md, (size_t)s.ld * s.nz * sizeof(float)));
+    HIPCHK(hipMalloc(&md, N * sizeof(float)));
     HIPCHK(hipMalloc(&ea, ne * 3 * sizeof(float)));
     HIPCHK(hipMalloc(&eb, ne * 3 * sizeof(float)));
     HIPCHK(hipMemsetAsync(dab, 0, N * 3 * sizeof(double), e->stream));

@@ -3,13 +3,16 @@
  *
  * Layout in HBM, per instance (slot):
  *   u[l]      displacement of time level l (3 levels rotating, 4 for
- *             two-step sweeps): x, y, z per node, 12 bytes; row stride ld
- *             nodes (ld >= nx+3, multiple of 64), so rows start on 128-byte
- *             lines
+ *             two-step sweeps): x, y, z per node, 12 bytes.  Nodes are
+ *             stored in column blocks of bw node columns (the owned columns
+ *             of one step-kernel tile, bw = (64/(hpn+1) - 1) * hpn), each
+ *             block row after row: node (ix, iz) at
+ *             (ix / bw) * bw * nz + iz * bw + ix % bw, so a wave sweeping a
+ *             tile upward reads and writes one contiguous run per plane
  *   comp[l]   one 32-bit word per node: the Kahan compensations of the three
  *             components at level l and at level l-1, signed 5-bit fields in
  *             units of ulp(u)/32 (zero without Kahan)
- *   mat_id    16-bit material record id per node (same row stride)
+ *   mat_id    16-bit material record id per node (same node order)
  *   emat      16-bit record id per element when all its nodes share one
  *             record, 0xFFFF otherwise
  *   mat       records {rho, 15 constants}, 16 floats each
@@ -33,7 +36,7 @@
 #define SEMSKS_QCOUNTERS (SEMSKS_MAXNP * SEMSKS_NQ)
 
 struct InstDesc {
-    int nex, nez, nx, nz, ld;
+    int nex, nez, nx, nz, bw;       /* bw: node columns per storage block */
     int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active, nlev;
     /* displacement of nlev (3, or 4 for two-step sweeps) time levels,
        level it in plane it % nlev: u[l] holds x, y, z per node (node q at
@@ -49,7 +52,7 @@ struct InstDesc {
     const float *mclass;
     const float *edge_a, *edge_b;   /* [n_edge][3] */
     const float *G, *Gv;            /* [nt_sem][2(nx+nz)], [nt_sem][nx+nz] */
-    const int *rec;                 /* node offsets iz*ld+ix */
+    const int *rec;                 /* node offsets of the receivers */
     const int *inj_rec;             /* per interface node: first receiver on it, or -1 */
     const int *rec_next;            /* per receiver: next one on its interface node (-1 at the
                                        end), -2 if it is not on an interface node */
