@@ -1,8 +1,9 @@
 """Edge cases of the inputs against the FP64 oracle: the smallest mesh the
 solver takes (2 x 2 elements, where the top-row Lysmer strip of 3*hpn nodes
 is longer than the row and the one tile holds two element columns),
-receivers on the first and last node columns, and a run of a single level
-(nst = 1: only GRT_Interface's level 0 is recorded)."""
+receivers on the first and last node columns, a run of a single level
+(nst = 1: only GRT_Interface's level 0 is recorded), and a mesh too large
+for 32-bit node offsets."""
 import os
 
 import numpy as np
@@ -79,3 +80,22 @@ def test_single_level(tmp_path, monkeypatch):
     ref = pyoracle.Oracle(["par=run.par"], "f64", cwd=d).run()
     assert tr.shape == ref.shape == (1, 3, 8)
     assert np.array_equal(tr.view(np.uint32), ref.view(np.uint32))
+
+
+def test_mesh_beyond_32bit_node_offsets_is_rejected():
+    """A mesh whose level plane needs more than 2^31 nodes is turned away
+    at admission with SEMSKS_EINVAL, before any of its arrays is read or
+    any device memory is taken; the engine then admits a normal case."""
+    print("case: admit a 60000x10000-element mesh (9.6e9 nodes)", flush=True)
+    import ctypes as C
+    mat = (C.c_float * semsks.MAT_STRIDE)(3.3, 200.0)
+    ids = (C.c_uint16 * 4)()
+    p = semsks.Problem(nex=60000, nez=10000, hpn=4, neh=1, eh=2.0, sdt=0.01, nst=10, solver=0, absorb=1,
+                       n_mat=1, mat_table=mat, mat_id=ids, ntrace=0, nt=0, dp=0.01)
+    with semsks.Engine() as e:
+        with pytest.raises(semsks.SemsksError) as err:
+            e.admit(p)
+        assert err.value.code == 1
+        from semsks import configs
+        b = configs.Built(inputs.small_case(nst=20))
+        e.admit(b.model.problem)
