@@ -1,7 +1,8 @@
 """Edge cases of the inputs against the FP64 oracle: the smallest mesh the
 solver takes (2 x 2 elements, where the top-row Lysmer strip of 3*hpn nodes
 is longer than the row and the one tile holds two element columns),
-receivers on the first and last node columns, a run of a single level
+receivers on the first and last node columns, meshes a whole number of
+storage blocks wide, a run of a single level
 (nst = 1: only GRT_Interface's level 0 is recorded), and a mesh too large
 for 32-bit node offsets."""
 import os
@@ -60,6 +61,26 @@ def test_receivers_on_the_side_columns_vs_fp64(tmp_path, monkeypatch):
     ix, _ = m.receivers()
     assert ix[0] == 0 and ix[1] >= m.problem.nex * m.problem.hpn - 1
     tr, ref = run_both(d)
+    check(tr, ref)
+
+
+@pytest.mark.parametrize("hpn,nex", [(4, 22), (3, 15)])
+def test_mesh_width_on_a_storage_block_boundary_vs_fp64(tmp_path, monkeypatch, hpn, nex):
+    """Nodes are stored in column blocks of one tile's owned width (11
+    elements at hpn = 4, 15 at hpn = 3).  A mesh a whole number of blocks
+    wide puts its last node column alone in a block of its own.  Receivers
+    on the first node column and next to the last one (whose force reads
+    that lone column), against the FP64 oracle."""
+    print("case: %dx12 elements, hpn %d, receivers at ix 0 and nx-2" % (nex, hpn), flush=True)
+    d = os.fspath(tmp_path)
+    inputs.small_case(nex=nex, hpn=hpn, nst=400, nrec=2, rec_x0=0.0, rec_x1=2.0 * nex, psi_deg=30.0,
+                      sh_zero=False).write(d)
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    ix, _ = m.receivers()
+    assert ix[0] == 0 and ix[1] >= m.problem.nex * m.problem.hpn - 1
+    tr, ref = run_both(d)
+    assert np.abs(ref).max() > 0.1
     check(tr, ref)
 
 
