@@ -24,3 +24,16 @@ def test_library_exports_every_declared_symbol(root):
 
 def test_cli_built():
     assert os.access(semsks.CLI_PATH, os.X_OK)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the built library the binding raises; there is no fallback."""
+    import subprocess
+    import sys
+    code = ("import semsks, sys\n"
+            "semsks.LIB_PATH = %r\n"
+            "try:\n    semsks.lib()\nexcept RuntimeError as e:\n    print('raised:', e); sys.exit(0)\n"
+            "sys.exit(1)\n") % os.fspath(tmp_path / "libsemsks.so")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, PYTHONPATH=os.pathsep.join(sys.path)))
+    assert r.returncode == 0 and "not built" in r.stdout, (r.stdout, r.stderr)
