@@ -46,6 +46,8 @@ def load(kind: str = "f64") -> C.CDLL:
     L.orc_set_node_tensor.argtypes = [vp, C.c_long, vp, C.c_float]
     L.orc_interp_series.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_float, vp, vp]
     L.orc_guard.argtypes = [vp, C.c_int]
+    L.orc_sponge.argtypes = [vp, vp]
+    L.orc_taper.argtypes = [vp, vp]
     L.orc_read_raymodel.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(C.c_int),
                                     C.POINTER(C.c_int)]
     _cache[kind] = L
@@ -119,6 +121,17 @@ class Oracle:
         f = np.zeros_like(u)
         self.L.orc_stiff(self.h, u.ctypes.data, f.ctypes.data)
         return np.moveaxis(f, -1, 0).copy()
+
+    def sponge(self, u_soa: np.ndarray) -> np.ndarray:
+        """atten_wave2 applied once to a field given as SoA planes [3][nz][nx]."""
+        u = np.ascontiguousarray(np.moveaxis(u_soa, 0, -1), dtype=self.dt)
+        self.L.orc_sponge(self.h, u.ctypes.data)
+        return np.moveaxis(u, -1, 0).copy()
+
+    def taper(self) -> np.ndarray:
+        out = np.zeros(self.neh * self.hpn, np.float32)
+        n = self.L.orc_taper(self.h, out.ctypes.data)
+        return out[:n]
 
     def update(self, u, u0, G):
         def aos(a):

@@ -1057,6 +1057,20 @@ ORC_EXPORT void orc_stiff(const orc_ctx *c, const real *u, real *f)
 {
     stiff_matrix(c, (orc_vec *)f, (const orc_vec *)u);
 }
+/* the edge sponge atten_wave2 applied once to a field (AoS), in place; a
+   no-op when atten = 0 */
+ORC_EXPORT void orc_sponge(const orc_ctx *c, real *u)
+{
+    if (c->atten > 0) atten_wave2(c, (orc_vec *)u);
+}
+/* the sponge factors atten_taper writes (neh*hpn values; none when atten = 0) */
+ORC_EXPORT int orc_taper(const orc_ctx *c, float *out)
+{
+    int i, nh = c->neh * c->hpn;
+    if (!c->atten) return 0;
+    for (i = 0; i < nh; i++) out[i] = c->taper[i];
+    return nh;
+}
 /* one explicit update from (u, u_prev, K u) including the absorbing rows */
 ORC_EXPORT void orc_update(const orc_ctx *c, const real *u, const real *u0, const real *G, real *u1)
 {

@@ -38,7 +38,23 @@ int main(int argc, char **argv)
     size_t n = (size_t)p->nst * 3 * (p->ntrace > 0 ? p->ntrace : 1);
     float *tr = (float *)calloc(n, sizeof(float));
     st = semsks_run_problem(p, tr);
-    if (st == SEMSKS_OK && dims[5]) st = semsks_write_seis(semsks_model_outdir(m), p->nst, p->ntrace, tr);
+    /* an unstable run still writes its traces, NaN/Inf included, as the
+       reference does (its fwrite in kirrecord_zhl, sem2d_psvsh.c:2116-2165,
+       never checks the values): a later uvw2sac step then converts this
+       event's output, not a stale seis.* of an earlier one */
+    if ((st == SEMSKS_OK || st == SEMSKS_ENONFINITE) && dims[5]) {
+        const semsks_status run_st = st;
+        char why[512];
+        snprintf(why, sizeof why, "%s", semsks_last_error());
+        st = semsks_write_seis(semsks_model_outdir(m), p->nst, p->ntrace, tr);
+        if (st == SEMSKS_OK && run_st != SEMSKS_OK) {
+            fprintf(stderr, "semsks: %s (seis.* written)\n", why);
+            st = run_st;
+            free(tr);
+            semsks_model_free(m);
+            return exit_code(st);
+        }
+    }
     if (st != SEMSKS_OK) fprintf(stderr, "semsks: %s\n", semsks_last_error());
     else printf("SEM calculation finished!\n");
     free(tr);

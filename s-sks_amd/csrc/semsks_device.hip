@@ -1697,6 +1697,7 @@ struct Slot {
     size_t bytes = 0;
     std::vector<void *> allocs;       /* owned device allocations */
     std::vector<uint16_t> emat;       /* host copy of the element-uniform record ids (owner) */
+    uint64_t mat_hash = 0;            /* material_hash of the problem (owner) */
     bool two_ok = false;              /* may take two steps per sweep */
 };
 
@@ -1817,6 +1818,7 @@ struct semsks_engine {
     /* device-resident plane-wave records waiting for the next admit */
     float *pend_psv = nullptr, *pend_sh = nullptr;
     int pend_nt = 0, pend_nx = 0, pend_nz = 0;
+    size_t pend_bytes = 0;            /* counted in used while they wait */
 };
 
 static int tile_rows(const semsks_engine *e, int nez)
@@ -2206,23 +2208,73 @@ static void build_schedule(int nlev, float rdt, int nstream, const int nrec[2], 
     }
 }
 
-/* device bytes a slot for p takes: its footprint, plus the two extra time
+/* device bytes a slot for p keeps: its footprint, plus the two extra time
    levels of the two-step sweep */
-static size_t admit_need(const semsks_engine *e, const semsks_problem *p, bool shared)
+static size_t admit_resident(const semsks_engine *e, const semsks_problem *p, bool shared)
 {
     return semsks_footprint(p, shared) + (e->opts.steps_per_sweep == 2 ? plane_of(p) * 16 : 0);
 }
 
-semsks_status semsks_room(semsks_engine *e, const semsks_problem *p)
+/* device bytes the admission of p holds only until semsks_admit returns:
+   the upload of host boundary records (none when they were generated on
+   the device, pending), the FP64 lumped mass and the small tables of the
+   one-off setup (none for a slot that shares its material), and the
+   interpolation schedule */
+static size_t admit_transient(const semsks_problem *p, bool shared, bool pending)
+{
+    const size_t mt = 2 * (size_t)(nx_of(p) + nz_of(p)), sht = (size_t)(nx_of(p) + nz_of(p));
+    size_t b = 0;
+    if (!pending && p->nt > 0) b += (size_t)p->nt * ((p->psv ? mt : 0) + (p->sh ? sht : 0)) * 4 + 512;
+    if (!shared) b += (size_t)nx_of(p) * nz_of(p) * 8 + 4096;
+    const int nlev = std::min(std::max(nt_sem_of(p), 1), std::max(p->nst, 1));
+    return b + (size_t)nlev * sizeof(SchedEntry) + 256;
+}
+
+/* device bytes of the plane-wave records semsks_planewave(..., NULL, NULL)
+   keeps for the next admission */
+static size_t pending_bytes(int nt, int nx, int nz) { return (size_t)nt * 3 * (nx + nz) * 4; }
+
+/* an allocation failure maps to ENOHBM (the serving loop then waits for
+   memory) when the device is out of memory, to EDEVICE otherwise; the
+   runtime's last-error state is cleared so the next check does not see it */
+static semsks_status alloc_failed(hipError_t err, const char *what)
+{
+    (void)hipGetLastError();
+    if (err == hipErrorOutOfMemory) return semsks_set_error(SEMSKS_ENOHBM, "device out of memory (%s)", what);
+    return semsks_set_error(SEMSKS_EDEVICE, "%s: %s", what, hipGetErrorString(err));
+}
+
+semsks_status semsks_room(semsks_engine *e, const semsks_problem *p, size_t pending)
 {
     if (!e || !p) return semsks_set_error(SEMSKS_EINVAL, "null argument");
     bool free_slot_left = false;
     for (auto &s : e->slots) free_slot_left = free_slot_left || !s.used;
     if (!free_slot_left) return semsks_set_error(SEMSKS_ENOSLOT, "all %zu slots busy", e->slots.size());
-    const size_t need = admit_need(e, p, false);
+    const size_t need = admit_resident(e, p, false) + admit_transient(p, false, pending > 0) + pending;
     if (e->used + need > e->budget)
         return semsks_set_error(SEMSKS_ENOHBM, "slot needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
     return SEMSKS_OK;
+}
+
+/* 64-bit hash of a problem's material (records and per-node ids): a slot
+   may share another's material only when the two hash alike */
+static uint64_t material_hash(const semsks_problem *p)
+{
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+    auto mix = [&h](const void *data, size_t n) {
+        const unsigned char *c = (const unsigned char *)data;
+        size_t i = 0;
+        for (; i + 8 <= n; i += 8) {
+            uint64_t v;
+            memcpy(&v, c + i, 8);
+            h = (h ^ v) * 0xff51afd7ed558ccdull;
+            h ^= h >> 33;
+        }
+        for (; i < n; ++i) h = (h ^ c[i]) * 0x100000001b3ull;
+    };
+    mix(p->mat_table, (size_t)p->n_mat * SEMSKS_MAT_STRIDE * sizeof(float));
+    mix(p->mat_id, (size_t)nx_of(p) * nz_of(p) * sizeof(uint16_t));
+    return h;
 }
 
 semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share, int *slot_out)
@@ -2244,8 +2296,12 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         if (q.nex != p->nex || q.nez != p->nez || q.hpn != p->hpn || q.eh != p->eh || q.sdt != p->sdt ||
             q.absorb != p->absorb)
             return semsks_set_error(SEMSKS_EINVAL, "share_slot %d has a different mesh", share);
+        if (e->slots[share].mat_hash != material_hash(p))
+            return semsks_set_error(SEMSKS_EINVAL, "share_slot %d has a different material", share);
     }
-    const size_t need = admit_need(e, p, share >= 0);
+    const bool pending = !p->psv && !p->sh && e->pend_psv && e->pend_nt == p->nt && e->pend_nx == nx_of(p) &&
+                         e->pend_nz == nz_of(p);
+    const size_t need = admit_resident(e, p, share >= 0) + admit_transient(p, share >= 0, pending);
     if (e->used + need > e->budget)
         return semsks_set_error(SEMSKS_ENOHBM, "slot needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
     for (int i = 0; i < p->ntrace; ++i)
@@ -2261,6 +2317,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     s.nt_sem = nt_sem_of(p);
     s.nlev = std::max(1, std::min(s.nt_sem, p->nst));
     s.share = share;
+    s.mat_hash = share >= 0 ? e->slots[share].mat_hash : material_hash(p);
     s.two_ok = true;
     for (int i = 0; i < p->ntrace; ++i)
         if (((p->rec_iz[i] == p->neh * p->hpn && p->rec_ix[i] >= p->neh * p->hpn) ||
@@ -2278,7 +2335,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     const size_t nodes = s.plane;
     const int mt = 2 * (s.nx + s.nz), sht = s.nx + s.nz;
     semsks_status st = SEMSKS_OK;
-#define ACHK(x) do { if ((x) != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "device allocation failed"); goto fail; } } while (0)
+#define ACHK(x) do { const hipError_t ae_ = (x); if (ae_ != hipSuccess) { st = alloc_failed(ae_, #x); goto fail; } } while (0)
     {
         float *st_all = nullptr;
         /* three levels (four for an engine that may take two steps per
@@ -2313,6 +2370,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     if (share >= 0) {
         const InstDesc &o = e->slots[share].d;
         d.mat_id = o.mat_id; d.emat = o.emat; d.mat = o.mat; d.mclass = o.mclass; d.edge_a = o.edge_a; d.edge_b = o.edge_b;
+        d.ortho = o.ortho;    /* the stress form follows the records the slot reads: the owner's */
         memcpy(d.Dg, o.Dg, sizeof d.Dg); memcpy(d.Dw, o.Dw, sizeof d.Dw); memcpy(d.cw, o.cw, sizeof d.cw);
         e->slots[share].refs++;
     } else {
@@ -2390,8 +2448,6 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         ACHK(hipMemsetAsync(Gv, 0, (size_t)s.nlev * sht * 4, e->stream));
         const float rdt = p->sdt / p->dp;
         const int nstream = p->solver == SEMSKS_SOLVER_PSVSH ? 2 : 1;
-        const bool pending = !p->psv && !p->sh && e->pend_psv && e->pend_nt == p->nt && e->pend_nx == s.nx &&
-                             e->pend_nz == s.nz;
         int nrec[2] = {(p->psv || pending) ? p->nt : 0, (p->sh || pending) ? p->nt : 0};
         std::vector<SchedEntry> sch[2];
         const int nlev_series = std::min(s.nt_sem, s.nlev);
@@ -2422,6 +2478,8 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         if (pending) {
             hipFree(e->pend_psv); hipFree(e->pend_sh);
             e->pend_psv = e->pend_sh = nullptr; e->pend_nt = 0;
+            e->used -= e->pend_bytes;
+            e->pend_bytes = 0;
         }
         d.G = G; d.Gv = Gv;
     }
@@ -2797,8 +2855,21 @@ semsks_status semsks_planewave(semsks_engine *e, int nex, int nez, int hpn, int 
         return semsks_set_error(SEMSKS_EINVAL, "bad plane-wave arguments");
     const int nx = nex * hpn + 1, nz = nez * hpn + 1, mt = 2 * (nx + nz), sht = nx + nz;
     float *dpsv = nullptr, *dsh = nullptr;
-    HIPCHK(hipMalloc(&dpsv, (size_t)nt * mt * 4));
-    HIPCHK(hipMalloc(&dsh, (size_t)nt * sht * 4));
+    if (!psv && !sh) {            /* records kept for the next admission: counted in the budget */
+        if (e->pend_psv) { hipFree(e->pend_psv); e->pend_psv = nullptr; }
+        if (e->pend_sh) { hipFree(e->pend_sh); e->pend_sh = nullptr; }
+        e->used -= e->pend_bytes;
+        e->pend_bytes = 0;
+        e->pend_nt = 0;
+        const size_t need = pending_bytes(nt, nx, nz);
+        if (e->used + need > e->budget)
+            return semsks_set_error(SEMSKS_ENOHBM, "plane-wave records need %zu bytes, %zu of %zu in use", need,
+                                    e->used, e->budget);
+    }
+    hipError_t ae = hipMalloc(&dpsv, (size_t)nt * mt * 4);
+    if (ae != hipSuccess) return alloc_failed(ae, "plane-wave records");
+    ae = hipMalloc(&dsh, (size_t)nt * sht * 4);
+    if (ae != hipSuccess) { hipFree(dpsv); return alloc_failed(ae, "plane-wave records"); }
     hipLaunchKernelGGL(planewave_kernel, dim3(2048), dim3(256), 0, e->stream, nex, nez, hpn, neh, (double)eh, nt,
                        (double)dp, p, beta, f0, psi, dpsv, dsh);
     HIPCHK(hipGetLastError());
@@ -2809,11 +2880,16 @@ semsks_status semsks_planewave(semsks_engine *e, int nex, int nez, int hpn, int 
         hipFree(dpsv); hipFree(dsh);
         return SEMSKS_OK;
     }
-    if (e->pend_psv) hipFree(e->pend_psv);
-    if (e->pend_sh) hipFree(e->pend_sh);
     e->pend_psv = dpsv; e->pend_sh = dsh; e->pend_nt = nt; e->pend_nx = nx; e->pend_nz = nz;
+    e->pend_bytes = pending_bytes(nt, nx, nz);
+    e->used += e->pend_bytes;
     HIPCHK(hipStreamSynchronize(e->stream));
     return SEMSKS_OK;
+}
+
+size_t semsks_planewave_bytes(int nex, int nez, int hpn, int nt)
+{
+    return pending_bytes(nt, nex * hpn + 1, nez * hpn + 1);
 }
 
 semsks_status semsks_run_problem(const semsks_problem *p, float *traces)

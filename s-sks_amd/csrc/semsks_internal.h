@@ -9,8 +9,14 @@ extern "C" {
 semsks_status semsks_set_error(semsks_status st, const char *fmt, ...);
 
 /* whether semsks_admit(e, p, -1, ...) would find a free slot and the HBM
-   for p: SEMSKS_OK, SEMSKS_ENOSLOT or SEMSKS_ENOHBM, with no side effect */
-semsks_status semsks_room(semsks_engine *e, const semsks_problem *p);
+   for p: SEMSKS_OK, SEMSKS_ENOSLOT or SEMSKS_ENOHBM, with no side effect.
+   pending > 0: p's records are first generated on the device by
+   semsks_planewave, which holds that many bytes until the admission */
+semsks_status semsks_room(semsks_engine *e, const semsks_problem *p, size_t pending);
+
+/* bytes semsks_planewave(e, ..., NULL, NULL) keeps on the device for nt
+   records of a mesh of nex x nez elements of degree hpn */
+size_t semsks_planewave_bytes(int nex, int nez, int hpn, int nt);
 
 /* getpar semantics (semsyn/src/getpar/src/getpar.c) for the library's
    readers: open = setpar, semsks_par_* = getpar (1 if the key is present),

@@ -39,7 +39,7 @@ static semsks_status try_admit(semsks_engine *e, semsks_request *r, int *slot)
     if (r->use_planewave) {
         /* the records are generated only for a request that fits now, not
            again on every round while it waits */
-        semsks_status st = semsks_room(e, p);
+        semsks_status st = semsks_room(e, p, semsks_planewave_bytes(p->nex, p->nez, p->hpn, p->nt));
         if (st != SEMSKS_OK) return st;
         st = semsks_planewave(e, p->nex, p->nez, p->hpn, p->neh, p->eh, p->nt, p->dp, r->planewave[0],
                                             r->planewave[1], r->planewave[2], r->planewave[3], NULL, NULL);
