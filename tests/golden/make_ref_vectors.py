@@ -5,7 +5,8 @@ oracle/Makefile from semsyn/src/interpl.c, readmodel.c and nrutil.c where
 they lie).  The vectors are data -- inputs and the reference's outputs --
 committed as ref_interpl.npz and ref_readmodel.npz.
 
-  python tests/golden/make_ref_vectors.py
+  python tests/golden/make_ref_vectors.py          (all files)
+  python tests/golden/make_ref_vectors.py mesh     (ref_interpl_mesh.npz only)
 """
 import ctypes as C
 import os
@@ -75,6 +76,64 @@ def ref_read_model(text, flat):
                 tth=arr(fp[3]), nen0=nen0, na=na, nrec=ip[2][nen0 - 2])
 
 
+# the mesh of the device test: 2 x 2 elements of degree 4 (nx = nz = 9), so
+# the record widths are the solver's own: mtotal = 2(nx+nz) = 36, shtotal = 18
+MESH_NX = MESH_NZ = 9
+MESH_SDT = "0.020000"
+MESH_RDT = (1.0, 0.999, 0.7, 0.37)
+
+
+def mesh_records(spiky):
+    """Boundary records for the device test: normal deviates (seeded), and in
+    the spiky set values the solver's guard must repair (12, -12, 15, NaN),
+    two of them at the first and last index of a record."""
+    rng = np.random.default_rng(20261204)
+    nrec = 40
+    psv = (2.0 * rng.standard_normal((nrec, 2 * (MESH_NX + MESH_NZ)))).astype(np.float32)
+    sh = (2.0 * rng.standard_normal((nrec, MESH_NX + MESH_NZ))).astype(np.float32)
+    np.clip(psv, -6, 6, out=psv)
+    np.clip(sh, -6, 6, out=sh)
+    if spiky:
+        psv[5, 0] = 12.0
+        psv[9, psv.shape[1] - 1] = -12.0
+        psv[14, 17] = np.nan
+        psv[21, 20] = 25.0
+        sh[7, 0] = np.nan
+        sh[11, sh.shape[1] - 1] = 15.0
+    return psv, sh
+
+
+def mesh_vectors():
+    """ref_interpl_mesh.npz: the reference interpl's vectors (before the
+    solver's guard) at the mesh widths, for dp = sdt / rdt written as the
+    par file holds it ("%.6f"), one stream (sem2d.c) and two streams sharing
+    the slot ring (sem2d_psvsh.c), plain and spiky records."""
+    out = {}
+    sdt = np.float32(float(MESH_SDT))
+    for spiky in (False, True):
+        psv, sh = mesh_records(spiky)
+        tag = "spiky" if spiky else "plain"
+        out[tag + "_psv"] = psv
+        out[tag + "_sh"] = sh
+        for rdt_want in MESH_RDT:
+            dp_txt = "%.6f" % (float(MESH_SDT) / rdt_want)
+            dp = np.float32(float(dp_txt))
+            rdt = np.float32(sdt / dp)                       # rdt = sdt/dp in float (sem2d_psvsh.c:388)
+            nt = psv.shape[0]
+            nt_sem = int(np.float32(np.float32((nt - 3) * dp) / sdt))
+            key = "%s_r%s" % (tag, ("%g" % rdt_want).replace(".", "p"))
+            out[key + "_dp"] = np.array(dp_txt)
+            out[key + "_ntsem"] = np.int32(nt_sem)
+            with np.errstate(invalid="ignore"):
+                shared = ref_interp_series([psv, sh], nt_sem, float(rdt), True)
+                single = ref_interp_series([psv], nt_sem, float(rdt), False)
+            out[key + "_shared_g"], out[key + "_shared_gv"] = shared
+            out[key + "_single_g"] = single[0]
+    out["sdt"] = np.array(MESH_SDT)
+    np.savez_compressed(os.path.join(HERE, "ref_interpl_mesh.npz"), **out)
+    print("wrote ref_interpl_mesh.npz", len(out), "arrays")
+
+
 def main():
     rng = np.random.default_rng(20261203)
     out = {}
@@ -114,4 +173,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["mesh"]:
+        mesh_vectors()
+    else:
+        main()
+        mesh_vectors()

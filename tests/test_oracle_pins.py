@@ -83,6 +83,32 @@ def test_interpolation_matches_reference_interpl():
             assert np.array_equal(o1.view(np.uint32), g[name + "_gv"].view(np.uint32)), name
 
 
+@pytest.mark.parametrize("solver", [0, 1])
+def test_boundary_series_at_mesh_widths_matches_reference(tmp_path, solver):
+    """The oracle's boundary series, read from the files through its own
+    par/greenfile reader, against the reference interpl's vectors at the
+    record widths of a 2x2-element mesh, dp = sdt/rdt, with the guard
+    applied per level (tests/golden/ref_interpl_mesh.npz)."""
+    from cases import interp_mesh_case
+    z = np.load(os.path.join(GOLD, "ref_interpl_mesh.npz"))
+    for tag in ("plain", "spiky"):
+        for rdt in ("1", "0p999", "0p7", "0p37"):
+            key = "%s_r%s" % (tag, rdt)
+            d = interp_mesh_case(tmp_path / key, tag, key, solver)
+            o = pyoracle.Oracle(["par=run.par"], "f32", cwd=d)
+            n = int(z[key + "_ntsem"])
+            assert o.nt_sem == n
+            g, gv = o.boundary_series(n)
+            want = z[key + ("_shared_g" if solver == 0 else "_single_g")]
+            want = np.stack([pyoracle.guard(r, "f32") for r in want])
+            assert np.array_equal(g.view(np.uint32), want.view(np.uint32)), key
+            if solver == 0:
+                wv = np.stack([pyoracle.guard(r, "f32") for r in z[key + "_shared_gv"]])
+                assert np.array_equal(gv.view(np.uint32), wv.view(np.uint32)), key
+            else:
+                assert not gv.any()
+
+
 def test_interpolation_delays_at_rdt1():
     """With dp = sdt, the single-stream solver injects record it-2 and the
     shared-ring solver record it-1 (interpl.c:59-91 with sem2d.c:410-413 /
