@@ -262,6 +262,19 @@ semsks_status semsks_run_problem(const semsks_problem *p, float *traces);
    element-force code as the time step. */
 semsks_status semsks_stiff(semsks_engine *e, int slot, const float *u, float *f);
 
+/* K u as semsks_stiff, with the field and the force in DEVICE memory
+   (u_dev, f_dev: [3][nz][nx] floats each, the caller's allocations), for
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  No counterpart in
+   the reference.  It allocates nothing: its scratch (the field and the force
+   in the kernels' node order, 24 bytes per node) is a fixed per-GPU device
+   array of SEMSKS_SCRATCH_BYTES (64 MiB, held until the process ends), so
+   meshes up to about 2.7 million nodes per plane; a larger mesh gets
+   SEMSKS_EINVAL.  Every call on one GPU uses that GPU's one array: calls
+   that overlap on one GPU (two engines on two host threads) race.  Returns
+   when f_dev holds the result. */
+semsks_status semsks_stiff_dev(semsks_engine *e, int slot, const float *u_dev, float *f_dev);
+
 /* Setup products of a slot, to host: lumped mass M (FP64, [nz][nx]), the
    FP32 time-loop factor dt^2/M ([nz][nx]), and the Lysmer coefficients A
    (FP64, [nz][nx][3], zero off the edges). */

@@ -62,6 +62,7 @@
 #include <string.h>
 #include <vector>
 #include <map>
+#include <mutex>
 #include <algorithm>
 
 #include "semsks.h"
@@ -1682,6 +1683,76 @@ __global__ void planewave_kernel(int nex, int nez, int n, int neh, double eh, in
 }
 
 /* ========================================================================= */
+/* fixed device scratch of semsks_stiff_dev                                  */
+/* ========================================================================= */
+/* The field and force of semsks_stiff_dev in the kernels' node order live
+   in this array, one copy per GPU, held until the process ends: an entry
+   point that allocates nothing.  Host code never passes the array's name
+   to a kernel (on the host it names a stand-in, not device memory); it uses
+   the per-device address from hipGetSymbolAddress (scratch_ptr). */
+static __device__ __align__(256) unsigned char scratch[SEMSKS_SCRATCH_BYTES];
+
+/* elements of the force-only tile height used by semsks_stiff_dev */
+constexpr int FORCE_TZ = 8;
+
+__global__ void fill_kernel(float *p, size_t n, float v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+/* SoA planes [3][nz][nx] -> xyz per node in the kernels' node order */
+__global__ void soa_to_blocks_kernel(int nx, int nz, int bw, const float *u, float *aos)
+{
+    const size_t N = (size_t)nx * nz;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < N; q += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(q / nx), ix = (int)(q - (size_t)iz * nx);
+        const long n = node_at(bw, nz, ix, iz);
+        aos[3 * n] = u[q];
+        aos[3 * n + 1] = u[N + q];
+        aos[3 * n + 2] = u[2 * N + q];
+    }
+}
+
+/* three force planes in the kernels' node order -> SoA planes [3][nz][nx] */
+__global__ void blocks_to_soa_kernel(int nx, int nz, int bw, size_t nodes, const float *f3, float *out)
+{
+    const size_t N = (size_t)nx * nz;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < N; q += (size_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(q / nx), ix = (int)(q - (size_t)iz * nx);
+        const long n = node_at(bw, nz, ix, iz);
+        out[q] = f3[n];
+        out[N + q] = f3[nodes + n];
+        out[2 * N + q] = f3[2 * nodes + n];
+    }
+}
+
+/* scratch layout for a plane of `nodes` nodes: u xyz at 0, the three force
+   planes after it (256-byte aligned); total bytes */
+static size_t scratch_f_offset(size_t nodes) { return (nodes * 12 + 255) / 256 * 256; }
+static size_t scratch_bytes(size_t nodes) { return scratch_f_offset(nodes) + nodes * 12; }
+
+#define SEMSKS_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[SEMSKS_MAX_DEVICES];
+
+/* device address of this GPU's scratch array, looked up once per device
+   with that device current (the engine's set-up) */
+static hipError_t scratch_ptr(int device, void **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= SEMSKS_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess) return err;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+/* ========================================================================= */
 /* engine                                                                    */
 /* ========================================================================= */
 namespace {
@@ -1698,6 +1769,12 @@ struct Slot {
     std::vector<void *> allocs;       /* owned device allocations */
     std::vector<uint16_t> emat;       /* host copy of the element-uniform record ids (owner) */
     uint64_t mat_hash = 0;            /* material_hash of the problem (owner) */
+    /* semsks_stiff_dev (owner): its force-only descriptor (field and force
+       in the scratch array), tile count and tiles, in one device block;
+       null when the mesh does not fit the scratch */
+    InstDesc *d_fdesc = nullptr;
+    int *d_fcount = nullptr;
+    int4 *d_ftiles = nullptr;
     bool two_ok = false;              /* may take two steps per sweep */
 };
 
@@ -1819,6 +1896,7 @@ struct semsks_engine {
     float *pend_psv = nullptr, *pend_sh = nullptr;
     int pend_nt = 0, pend_nx = 0, pend_nz = 0;
     size_t pend_bytes = 0;            /* counted in used while they wait */
+    void *scratch = nullptr;          /* this device's scratch array (semsks_stiff_dev) */
 };
 
 static int tile_rows(const semsks_engine *e, int nez)
@@ -2085,6 +2163,7 @@ extern "C" {
 static semsks_status engine_init(semsks_engine *e)
 {
     HIPCHK(hipGetDevice(&e->device));
+    HIPCHK(scratch_ptr(e->device, &e->scratch));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, e->device));
     e->n_cu = prop.multiProcessorCount;
@@ -2168,7 +2247,8 @@ size_t semsks_footprint(const semsks_problem *p, int shared)
     size_t b = nodes * 4 * (3 * 4);                 /* three levels: xyz plane + compensation words */
     if (!shared) b += nodes * 2 + (size_t)p->nex * p->nez * 2 +
                       (size_t)p->n_mat * (64 + 4 * (p->hpn + 1) * (p->hpn + 1)) +
-                      (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24;
+                      (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24 +
+                      2048 + (size_t)(p->nex / 2 + 1) * (p->nez + 1) * 16;   /* semsks_stiff_dev tiles (bound) */
     const int mt = 2 * (nx_of(p) + nz_of(p)), sht = nx_of(p) + nz_of(p);
     int nlev = std::min(std::max(nt_sem_of(p), 1), std::max(p->nst, 1));
     b += (size_t)nlev * (mt + sht) * 4;
@@ -2438,6 +2518,28 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
             hipFree(dd); hipFree(w64); hipFree(mass); hipFree(dev);
             if (!(maxdev <= 1.e-2)) { st = semsks_set_error(SEMSKS_EINVAL, "non-uniform Jacobian (max dev %g)", maxdev); goto fail; }
         }
+    }
+    /* the force-only descriptor and tiles of semsks_stiff_dev (owners whose
+       plane fits the scratch array) */
+    if (share < 0 && scratch_bytes(nodes) <= SEMSKS_SCRATCH_BYTES) {
+        std::vector<int4> ft;
+        build_tiles(s, 0, std::min(FORCE_TZ, std::max(1, p->nez)), false, ft);
+        InstDesc fd = d;
+        fd.active = 1;
+        unsigned char *base = (unsigned char *)e->scratch;
+        fd.u[0] = (float *)base;
+        for (int c = 0; c < 3; ++c) fd.fout[c] = (float *)(base + scratch_f_offset(nodes)) + (size_t)c * nodes;
+        unsigned char *blk = nullptr;
+        const int nft = (int)ft.size();
+        const size_t dsz = (sizeof(InstDesc) + 255) / 256 * 256;     /* [desc | count | tiles] */
+        ACHK(dalloc(s, &blk, dsz + 256 + ft.size() * sizeof(int4)));
+        s.d_fdesc = (InstDesc *)blk;
+        s.d_fcount = (int *)(blk + dsz);
+        s.d_ftiles = (int4 *)(blk + dsz + 256);
+        ACHK(hipMemcpyAsync(s.d_fdesc, &fd, sizeof fd, hipMemcpyHostToDevice, e->stream));
+        ACHK(hipMemcpyAsync(s.d_fcount, &nft, sizeof nft, hipMemcpyHostToDevice, e->stream));
+        if (nft) ACHK(hipMemcpyAsync(s.d_ftiles, ft.data(), ft.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
+        ACHK(hipStreamSynchronize(e->stream));
     }
     /* boundary series */
     {
@@ -2795,6 +2897,30 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
         for (int iz = 0; iz < s.nz; ++iz)
             for (int ix = 0; ix < s.nx; ++ix)
                 f[((size_t)c * s.nz + iz) * s.nx + ix] = fb[c * nodes + node_at(s.bw, s.nz, ix, iz)];
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_stiff_dev(semsks_engine *e, int k, const float *u_dev, float *f_dev)
+{
+    if (!e || k < 0 || k >= (int)e->slots.size() || !e->slots[k].used) return semsks_set_error(SEMSKS_EINVAL, "bad slot %d", k);
+    if (!u_dev || !f_dev) return semsks_set_error(SEMSKS_EINVAL, "null field or force pointer");
+    const Slot &s = e->slots[k];
+    const Slot &o = s.share >= 0 ? e->slots[s.share] : s;
+    const size_t nodes = s.plane;
+    if (!o.d_fdesc || !e->scratch)
+        return semsks_set_error(SEMSKS_EINVAL, "mesh of %zu nodes per plane needs %zu bytes of scratch, limit %zu",
+                                nodes, scratch_bytes(nodes), (size_t)SEMSKS_SCRATCH_BYTES);
+    unsigned char *base = (unsigned char *)e->scratch;
+    float *ub = (float *)base, *fb = (float *)(base + scratch_f_offset(nodes));
+    /* the scratch keeps the previous call's contents: clear both regions
+       (fill kernels, so the calls could be captured in a graph) */
+    hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, e->stream, ub, nodes * 3, 0.f);
+    hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, e->stream, fb, nodes * 3, 0.f);
+    hipLaunchKernelGGL(soa_to_blocks_kernel, dim3(1024), dim3(256), 0, e->stream, s.nx, s.nz, s.bw, u_dev, ub);
+    launch_tiles<true>(e, s.meta.hpn, o.d_fdesc, e->d_levels, o.d_ftiles, o.d_fcount);
+    hipLaunchKernelGGL(blocks_to_soa_kernel, dim3(1024), dim3(256), 0, e->stream, s.nx, s.nz, s.bw, nodes, fb, f_dev);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
     return SEMSKS_OK;
 }
 

@@ -35,6 +35,11 @@
 #define SEMSKS_QSTRIDE 32
 #define SEMSKS_QCOUNTERS (SEMSKS_MAXNP * SEMSKS_NQ)
 
+/* bytes of the fixed per-GPU scratch array of semsks_stiff_dev: the field
+   and the force of one plane, 24 bytes per node (with padding), so meshes
+   of up to about 2.7 million nodes per plane */
+#define SEMSKS_SCRATCH_BYTES ((size_t)64 << 20)
+
 struct InstDesc {
     int nex, nez, nx, nz, bw;       /* bw: node columns per storage block */
     int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active, nlev;

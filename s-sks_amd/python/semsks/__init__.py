@@ -29,7 +29,7 @@ EXPORTS = [
     "semsks_model_free", "semsks_write_seis", "semsks_last_error", "semsks_engine_create",
     "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_advance",
     "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_retire", "semsks_sync",
-    "semsks_last_timing", "semsks_last_timed_steps", "semsks_sweep_steps", "semsks_tile_rows", "semsks_run_problem", "semsks_stiff",
+    "semsks_last_timing", "semsks_last_timed_steps", "semsks_sweep_steps", "semsks_tile_rows", "semsks_run_problem", "semsks_stiff", "semsks_stiff_dev",
     "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve", "semsks_uvw2sac",
 ]
 
