@@ -92,6 +92,7 @@ def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0):
     log("  admitted %d instances in %.1fs" % (len(waves), time.time() - t0))
     npts = sum(b.nglob() for b, _, _ in waves)
     eng.advance(warmup, sync=True)
+    eng.prepare(steps)              # the graphs the timed call replays, captured outside the timed region
     eng.sync()
     t0 = time.perf_counter()
     eng.advance(steps, sync=True)
@@ -101,10 +102,16 @@ def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0):
                 tile_rows=eng.tile_rows())
 
 
-def run_serving(rate, duration, nex_range, nez_range, nst_range, budget_gb=200.0, max_slots=1024, seed=None):
+def run_serving(rate, duration, nex_range, nez_range, nst_range, budget_gb=200.0, max_slots=1024, seed=None,
+                check_sample=True):
     """Continuous batching of Poisson-arriving C4-style requests
     (semsks_serve): HBM-footprint admission against budget_gb, all admitted
-    instances stepped together, each retired at its own last level."""
+    instances stepped together, each retired at its own last level.  Reports
+    latency (arrival to traces on the host), admission wait, the offered
+    load against what the GPU served, the backlog left when arrivals stop,
+    and, for the seeded 5% sample (configs.c5_sample) that has FP64
+    fixtures (tests/golden/c5_sample_fp64_traces.npz), each sampled
+    request's FP32-vs-FP64 misfit."""
     import semsks
     from semsks import configs
     kw = {} if seed is None else {"seed": seed}
@@ -122,15 +129,57 @@ def run_serving(rate, duration, nex_range, nez_range, nst_range, budget_gb=200.0
     budget = int(budget_gb * 1e9)
     with semsks.Engine(max_slots=max_slots, hbm_budget=budget) as e:
         res, st = e.serve(probs, [a for a, _ in reqs], pws, steps_per_round=16, want_traces=True)
-    lat = np.array([r["latency"] for r in res if r["status"] == 0])
-    wait = np.array([r["admit"] - r["arrival"] for r in res if r["status"] == 0])
-    return {"requests": len(reqs), "completed": st["completed"], "rate_per_s": rate, "arrival_window_s": duration,
-            "wall_s": st["wall_s"], "throughput": st["point_steps"] / st["wall_s"],
-            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-            "p50_admission_wait_s": float(np.percentile(wait, 50)), "max_active": st["max_active"],
-            "hbm_budget_gb": budget_gb,
-            "mesh_elements": "%dx%d..%dx%d" % (nex_range[0], nez_range[0], nex_range[1], nez_range[1]),
-            "steps": "%d..%d" % nst_range}
+    ok = [r for r in res if r["status"] == 0]
+    lat = np.array([r["latency"] for r in ok])
+    wait = np.array([r["admit"] - r["arrival"] for r in ok])
+    work = np.array([b.nglob() * (b.case.nst - 1) for b in built], np.float64)
+    arrivals = np.array([a for a, _ in reqs])
+    done = np.array([r["done"] for r in res])
+    in_system = (arrivals <= duration) & (done > duration)
+    out = {"requests": len(reqs), "completed": st["completed"], "rate_per_s": rate, "arrival_window_s": duration,
+           "wall_s": st["wall_s"], "throughput": st["point_steps"] / st["wall_s"],
+           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+           "p50_admission_wait_s": float(np.percentile(wait, 50)),
+           "p99_admission_wait_s": float(np.percentile(wait, 99)), "max_active": st["max_active"],
+           "offered_point_steps_per_s": float(work.sum() / duration),
+           "load_over_served": float(work.sum() / duration / (st["point_steps"] / st["wall_s"])),
+           "backlog_at_window_end_requests": int(in_system.sum()),
+           "backlog_at_window_end_point_steps": float(work[in_system].sum()),
+           "drain_after_window_s": float(done.max() - duration),
+           "hbm_budget_gb": budget_gb,
+           "mesh_elements": "%dx%d..%dx%d" % (nex_range[0], nez_range[0], nex_range[1], nez_range[1]),
+           "steps": "%d..%d" % nst_range}
+    if check_sample and seed is None:
+        out["fp64_sample"] = serving_sample_check(reqs, res)
+    return out
+
+
+def serving_sample_check(reqs, res):
+    """FP32 traces the serving loop returned for the seeded 5% sample against
+    the FP64 oracle's (tests/golden/c5_sample_fp64_traces.npz, every
+    decim-th level): worst rel-L2 and peak misfit per request and component."""
+    from semsks import configs
+    path = os.path.join(ROOT, "tests", "golden", "c5_sample_fp64_traces.npz")
+    sample = configs.c5_sample(len(reqs))
+    if not os.path.exists(path):
+        return {"sampled": len(sample), "checked": 0, "note": "no fixture"}
+    ref = np.load(path)
+    dec = int(ref["decim"])
+    rows, worst = [], 0.0
+    for i in sample:
+        if "r%d_u" % i not in ref.files or res[i]["status"] != 0:
+            continue
+        tr = res[i]["traces"][::dec]
+        row = {"request": i}
+        for name, k, key in (("u", 0, "u"), ("w", 1, "mw"), ("v", 2, "v")):
+            a, b = tr[:, k].astype(np.float64), ref["r%d_%s" % (i, key)].astype(np.float64)
+            l2 = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+            pk = np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)
+            row[name] = [float(l2.max()), float(pk.max())]
+            worst = max(worst, l2.max(), pk.max())
+        rows.append(row)
+    return {"sampled": len(sample), "checked": len(rows), "worst": worst, "per_request": rows,
+            "reference": "FP64 oracle (tests/golden/c5_sample_fp64_traces.npz, every %dth level)" % dec}
 
 
 def precision_vs_fixture(case, fixture, label):
@@ -187,6 +236,63 @@ def precision_aniso():
                                 "elements, coupled P-SV+SH, plane S 30 deg off SV, p=0.05, 5000 steps, 50 receivers")
 
 
+def precision_aniso20k():
+    """The same model at the coupled solver's production length, 20000
+    steps, against tests/golden/aniso20k_fp64_traces.npz (every 5th level;
+    tests/golden/make_aniso20k_fp64.py)."""
+    import tempfile
+    import semsks
+    from semsks import configs
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "aniso20k_fp64_traces.npz"))
+    nst, dec = int(ref["nst"]), int(ref["decim"])
+    old = os.getcwd()
+    with tempfile.TemporaryDirectory() as d:
+        configs.c2_model_c1_mesh(nst=nst).write(d)
+        os.chdir(d)
+        try:
+            m = semsks.Model(["par=run.par"])
+            with semsks.Engine() as e:
+                s = e.admit(m.problem)
+                e.advance(m.problem.nst - 1)
+                tr = e.traces(s)[::dec]
+        finally:
+            os.chdir(old)
+    out = {"case": "C2 model on 200x100 elements, coupled P-SV+SH, %d steps (every %dth level compared), "
+                   "50 receivers" % (nst, dec),
+           "reference": "FP64 oracle (tests/golden/aniso20k_fp64_traces.npz)", "seismograms": 0}
+    for name, k, key in (("u", 0, "u"), ("w", 1, "mw"), ("v", 2, "v")):
+        a, b = tr[:, k].astype(np.float64), ref[key].astype(np.float64)
+        l2 = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+        pk = np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)
+        out[name] = {"rel_l2_max": float(l2.max()), "rel_l2_median": float(np.median(l2)),
+                     "peak_misfit_max": float(pk.max()), "peak_misfit_median": float(np.median(pk))}
+        out["seismograms"] += int(tr.shape[2])
+    return out
+
+
+def run_c4(steps, warmup, tile_z=0, kernel_steps=10, n=96):
+    """The uneven batch: 96 seeded C4 instances (200x100..1600x400 elements,
+    1-4 layers, iso or tilted hexagonal up to 6%), one persistent launch per
+    step.  Bytes: each instance's own 36 + 4 (n_C + 1) per point-step (no
+    material shared), n_C = 15 with a tilted layer, else 9."""
+    from semsks import configs
+    cases = configs.c4_cases(n)
+    t0 = time.time()
+    built = [configs.Built(c) for c in cases]
+    log("  c4: built %d meshes in %.1fs (%.3g GLL points)" % (len(built), time.time() - t0,
+                                                              sum(b.nglob() for b in built)))
+    waves = [(b, b.case.p, b.case.psi_deg) for b in built]
+    r4 = run_engine(waves, steps, warmup, False, tile_z=tile_z, extra_steps=kernel_steps)
+    k4, _ = kernel_time(r4["engine"], kernel_steps)
+    r4["engine"].close()
+    npts = r4["npts"]
+    n_c = [15 if any(a is not None and a[1] is not None for a in (b.case.aniso or [])) else 9 for b in built]
+    B = sum((36.0 + 4.0 * (nc + 1)) * b.nglob() for nc, b in zip(n_c, built)) / npts
+    return dict(throughput=r4["throughput"], ms_per_step=r4["ms_per_step"], kernel_ms=k4, npts=int(npts),
+                tile_rows=r4["tile_rows"], bytes_per_point_step_alg=B,
+                achieved_GBs=B * npts / (k4 * 1e-3) / 1e9)
+
+
 def kernel_time(eng, nsteps):
     """Average device time of the fused step kernel over nsteps eager steps."""
     import semsks
@@ -195,29 +301,34 @@ def kernel_time(eng, nsteps):
     return kern / max(n, 1), total / nsteps
 
 
-def cpu_baseline(case_builder, steps: int, threads: int | None = None, kind: str = "f32"):
+def cpu_threads():
+    """Threads of the CPU baseline: OMP_NUM_THREADS when set (the CPU share
+    a GPU box gives one GPU), else every core this process may run on."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return env or len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(case_builder, steps: int, warmup: int, kind: str = "f32"):
     """Faithful FP32 port on the host, one instance of the mesh: kind "f32"
     is the -O3 build, "f32_O0" the build with the reference makefile's own
-    flags (cc -fopenmp, no -O)."""
+    flags (cc -fopenmp, no -O).  The rate of a window of `steps` steps after
+    `warmup` steps, from the oracle's per-step clock."""
     import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    from semsks import inputs
-    if threads:
-        os.environ["OMP_NUM_THREADS"] = str(threads)
     c = case_builder()
+    n = warmup + steps + 1                       # level 0 is set up, not stepped
     with tempfile.TemporaryDirectory() as d:
-        c.nst = steps + 1
-        c.write(d, nt=steps + 8)
+        c.nst = n
+        c.write(d, nt=n + 7)
         t0 = time.time()
         o = pyoracle.Oracle(["par=run.par"], kind, cwd=d)
         setup = time.time() - t0
-        t0 = time.perf_counter()
-        o.run(steps + 1)
-        wall = time.perf_counter() - t0
+        _, clock = o.run_timed(n)
         nglob = o.nx * o.nz
         o.close()
-    return dict(value=nglob * steps / wall, steps=steps, wall=wall, setup=setup, nglob=nglob)
+    wall = clock[n - 1] - clock[warmup]
+    return dict(value=nglob * steps / wall, steps=steps, warmup=warmup, wall=wall, setup=setup, nglob=nglob)
 
 
 def main():
@@ -231,13 +342,17 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=16, help="timed steps of the CPU baseline (-O3 build)")
+    ap.add_argument("--cpu-warmup", type=int, default=10, help="untimed steps before them")
     ap.add_argument("--cpu-steps-o0", type=int, default=2,
-                    help="steps of the port built with the reference makefile's flags (0 = skip)")
-    ap.add_argument("--serve-window", type=float, default=5.0,
-                    help="seconds of Poisson arrivals in the serving leg of the default run (0 = skip)")
-    ap.add_argument("--only", choices=["plain", "batched"], default=None,
-                    help="c3: run only one of the two legs (for profiler runs)")
+                    help="timed steps of the port built with the reference makefile's flags (0 = skip)")
+    ap.add_argument("--cpu-warmup-o0", type=int, default=1)
+    ap.add_argument("--serve-window", type=float, default=3.0,
+                    help="seconds of Poisson arrivals of C5-size requests in the serving leg of the default "
+                         "run (0 = skip)")
+    ap.add_argument("--no-c4", dest="c4", action="store_false", help="skip the uneven-batch (C4) record")
+    ap.add_argument("--only", choices=["batched"], default=None,
+                    help="c3: run only the batched leg (for profiler runs)")
     ap.add_argument("--kernel-steps", type=int, default=10)
     ap.add_argument("--steps-per-sweep", type=int, default=1, choices=[1, 2],
                     help="2: two time steps per sweep of the mesh (step2_kernel) where possible")
@@ -247,6 +362,9 @@ def main():
                     help="only time a device copy and a triad a = b + 2c over arrays of this many MiB "
                          "(the practical HBM rate for 1:1 and 2:1 read:write traffic)")
     args = ap.parse_args()
+    # the CPU baseline's OpenMP threads stay on neighbouring cores; set before
+    # anything loads the OpenMP runtime
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     global STEPS_PER_SWEEP
     STEPS_PER_SWEEP = args.steps_per_sweep
     if args.copy_calibration:
@@ -289,19 +407,6 @@ def main():
 
     out = {}
     if args.config == "c3":
-        # plain: one instance of the production anisotropic model (C2)
-        r2, k2 = {"throughput": float("nan"), "ms_per_step": float("nan")}, float("nan")
-        if args.only != "batched":
-            log("plain: C2 mesh, 1 instance")
-            b2 = configs.Built(configs.c2_case(nst=10))
-            r2 = run_engine([(b2, 0.05, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z,
-                            extra_steps=args.kernel_steps)
-            k2, _ = kernel_time(r2["engine"], args.kernel_steps)
-            r2["engine"].close()
-            log("  plain %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms, tiles of %d rows" % (
-                r2["throughput"], r2["ms_per_step"], k2, r2["tile_rows"]))
-            if args.only == "plain":
-                return
         # batched: 64 waves on the C2 mesh with the isotropic model, one material
         log("batched: C3, %d instances" % args.batch)
         b3 = configs.Built(configs.c3_case(nst=10))
@@ -311,51 +416,86 @@ def main():
         r3["engine"].close()
         log("  batched %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms, tiles of %d rows" % (
             r3["throughput"], r3["ms_per_step"], k3, r3["tile_rows"]))
+        if args.only == "batched":
+            return
+        # plain, like for like: one instance of the same workload (same mesh,
+        # model and kernels, batch 1)
+        log("plain: 1 instance of the C3 workload")
+        rp = run_engine(waves[:1], args.steps, args.warmup, False, tile_z=args.tile_z, extra_steps=args.kernel_steps)
+        kp, _ = kernel_time(rp["engine"], args.kernel_steps)
+        rp["engine"].close()
+        log("  plain %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms, tiles of %d rows" % (
+            rp["throughput"], rp["ms_per_step"], kp, rp["tile_rows"]))
+        # the production anisotropic model (C2) alone
+        log("C2: 1 instance, tilted-axis anisotropic model")
+        b2 = configs.Built(configs.c2_case(nst=10))
+        r2 = run_engine([(b2, 0.05, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z,
+                        extra_steps=args.kernel_steps)
+        k2, _ = kernel_time(r2["engine"], args.kernel_steps)
+        r2["engine"].close()
+        log("  C2 %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms, tiles of %d rows" % (
+            r2["throughput"], r2["ms_per_step"], k2, r2["tile_rows"]))
         # the same two legs on the 200x100 mesh (321,201 points per instance)
-        r1p = r1b = None
-        if args.only is None:
-            log("plain and batched on the C1 mesh")
-            b1 = configs.Built(configs.c1_case(nst=10, solver=0, sh_zero=False))
-            r1p = run_engine([(b1, 0.06, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z)
-            r1p["engine"].close()
-            w1 = [(b1, p, baz) for p, baz in configs.c3_waves()][: args.batch]
-            r1b = run_engine(w1, args.steps, args.warmup, True, tile_z=args.tile_z)
-            r1b["engine"].close()
-            log("  C1 plain %.3e (tiles of %d rows), batched x%d %.3e pt-steps/s (%d rows)" % (
-                r1p["throughput"], r1p["tile_rows"], len(w1), r1b["throughput"], r1b["tile_rows"]))
+        log("plain and batched on the C1 mesh")
+        b1 = configs.Built(configs.c1_case(nst=10, solver=0, sh_zero=False))
+        r1p = run_engine([(b1, 0.06, 0.0)], args.steps, args.warmup, False, tile_z=args.tile_z)
+        r1p["engine"].close()
+        w1 = [(b1, p, baz) for p, baz in configs.c3_waves()][: args.batch]
+        r1b = run_engine(w1, args.steps, args.warmup, True, tile_z=args.tile_z)
+        r1b["engine"].close()
+        log("  C1 plain %.3e (tiles of %d rows), batched x%d %.3e pt-steps/s (%d rows)" % (
+            r1p["throughput"], r1p["tile_rows"], len(w1), r1b["throughput"], r1b["tile_rows"]))
         B = alg_bytes(9, len(waves))
         achieved = B * r3["npts"] / (k3 * 1e-3) / 1e9
         value, ms, npts, steps = r3["throughput"], r3["ms_per_step"], r3["npts"], args.steps
+        hbm_bound_plain = HBM_PEAK_GBS * 1e9 / alg_bytes(9, 1)
         cfg = {"workload": "uniform batch: %d plane S waves (16 baz x 4 p) on a 2000x500-element NGLL=5 mesh, "
                            "16,010,001 GLL points each, shared isotropic 3-layer model" % len(waves),
                "instances": len(waves), "nglob_per_instance": int(b3.nglob()), "mesh": "2000x500 elements",
-               "plain_workload": "1 instance, 2000x500 mesh, tilted-axis hexagonal 4% upper mantle (15 constants)",
-               "plain_value": r2["throughput"], "plain_ms_per_step": r2["ms_per_step"],
-               "batched_over_plain": r3["throughput"] / r2["throughput"],
-               "kernel_ms_batched": k3, "kernel_ms_plain": k2,
-               "tile_rows_batched": r3["tile_rows"], "tile_rows_plain": r2.get("tile_rows"),
+               "plain_workload": "1 instance of the same workload (same mesh, model and kernels, batch 1)",
+               "plain_value": rp["throughput"], "plain_ms_per_step": rp["ms_per_step"],
+               "batched_over_plain": r3["throughput"] / rp["throughput"],
+               "batched_over_plain_note": "on the 16M-point mesh one instance already fills the GPU: 5x the plain "
+                                          "rate would need %.1f TB/s at the batch's 36.6 B/pt, over the 8 TB/s HBM "
+                                          "peak; the ratio on the 200x100 mesh is c1_mesh_batched_over_plain"
+                                          % (5 * rp["throughput"] * B / 1e12),
+               "plain_hbm_bound": hbm_bound_plain,
+               "kernel_ms_batched": k3, "kernel_ms_plain": kp,
+               "tile_rows_batched": r3["tile_rows"], "tile_rows_plain": rp["tile_rows"],
+               "c2_workload": "1 instance, 2000x500 mesh, tilted-axis hexagonal 4% upper mantle (15 constants)",
+               "c2_value": r2["throughput"], "c2_ms_per_step": r2["ms_per_step"], "kernel_ms_c2": k2,
+               "c1_mesh_plain_value": r1p["throughput"], "c1_mesh_batched_value": r1b["throughput"],
+               "c1_mesh_tile_rows_plain": r1p["tile_rows"], "c1_mesh_tile_rows_batched": r1b["tile_rows"],
+               "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
+               "c1_mesh_workload": "64 plane S waves on the 200x100-element mesh vs one",
                "bytes_per_point_step_alg": B, "kahan": True}
-        if args.only is None and args.serve_window > 0:
-            log("serving: Poisson 20/s for %.0f s, scaled C4-style requests" % args.serve_window)
-            sv = run_serving(20.0, args.serve_window, (200, 800), (100, 200), (500, 2000))
-            log("  %d requests, p50 %.3f s, p99 %.3f s, %.3e pt-steps/s" % (sv["completed"], sv["p50_latency_s"],
-                                                                         sv["p99_latency_s"], sv["throughput"]))
+        if args.c4:
+            log("c4: uneven batch of 96 instances")
+            c4 = run_c4(args.steps, args.warmup, tile_z=args.tile_z, kernel_steps=args.kernel_steps)
+            c4["traffic_bytes_per_launch"] = profiled_traffic("step_kernel_uneven96_c4.md")
+            if c4["traffic_bytes_per_launch"]:
+                c4["traffic_GBs"] = c4["traffic_bytes_per_launch"] / (c4["kernel_ms"] * 1e-3) / 1e9
+            c4["workload"] = ("uneven batch: 96 instances, 200x100..1600x400 elements, 1-4 layers, iso or tilted "
+                              "hexagonal up to 6%, one persistent kernel per step")
+            log("  c4 %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms" % (c4["throughput"], c4["ms_per_step"],
+                                                                      c4["kernel_ms"]))
+            cfg["c4"] = c4
+        if args.serve_window > 0:
+            log("serving: Poisson 20/s for %.0f s, C5 requests" % args.serve_window)
+            sv = run_serving(20.0, args.serve_window, (200, 1600), (100, 400), (2000, 12000))
+            log("  %d of %d requests done, p50 %.3f s, p99 %.3f s, %.3e pt-steps/s" % (
+                sv["completed"], sv["requests"], sv["p50_latency_s"], sv["p99_latency_s"], sv["throughput"]))
             cfg["serving"] = sv
-        if args.only is None:
-            log("precision: 200x100 case, 4000 steps, FP32 GPU vs FP64 oracle")
-            cfg["precision"] = precision_c1()
-            cfg["precision_aniso"] = precision_aniso()
-            log("  u rel-L2 max %.2e, w rel-L2 max %.2e" % (cfg["precision"]["u"]["rel_l2_max"],
-                                                         cfg["precision"]["w"]["rel_l2_max"]))
-            pa = cfg["precision_aniso"]
-            log("  anisotropic P-SV+SH case: u %.2e, w %.2e, v %.2e worst rel-L2" % (
-                pa["u"]["rel_l2_max"], pa["w"]["rel_l2_max"], pa["v"]["rel_l2_max"]))
-        if r1p is not None:
-            cfg.update({"c1_mesh_plain_value": r1p["throughput"], "c1_mesh_batched_value": r1b["throughput"],
-                    "c1_mesh_tile_rows_plain": r1p["tile_rows"], "c1_mesh_tile_rows_batched": r1b["tile_rows"],
-                        "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
-                        "c1_mesh_workload": "64 plane S waves on the 200x100-element mesh vs one"})
-        cpu_case = lambda: configs.c2_case(nst=10)
+        log("precision: FP32 GPU vs FP64 oracle fixtures")
+        cfg["precision"] = precision_c1()
+        cfg["precision_aniso"] = precision_aniso()
+        cfg["precision_aniso_20000"] = precision_aniso20k()
+        for key in ("precision", "precision_aniso", "precision_aniso_20000"):
+            pa = cfg[key]
+            log("  %s: u %.2e, w %.2e, v %s worst rel-L2" % (key, pa["u"]["rel_l2_max"], pa["w"]["rel_l2_max"],
+                                                           pa["v"] if isinstance(pa["v"], str) else
+                                                           "%.2e" % pa["v"]["rel_l2_max"]))
+        cpu_case = lambda: configs.c3_case(nst=10)
         traffic = profiled_traffic("step_kernel_batched64_2000x500.md") if len(waves) == 64 else None
     elif args.config == "c2":
         b2 = configs.Built(configs.c2_case(nst=10))
@@ -372,31 +512,24 @@ def main():
     elif args.config == "c4":
         # uneven batch: 96 instances, 200x100 .. 1600x400 elements, 1-4 layers
         # iso/hexagonal, all stepped by one persistent launch per step
-        cases = configs.c4_cases(96)
-        t0 = time.time()
-        built = [configs.Built(c) for c in cases]
-        log("c4: built %d meshes in %.1fs (%.3g GLL points)" % (len(built), time.time() - t0,
-                                                                  sum(b.nglob() for b in built)))
-        waves = [(b, b.case.p, b.case.psi_deg) for b in built]
-        r4 = run_engine(waves, args.steps, args.warmup, False, tile_z=args.tile_z, extra_steps=args.kernel_steps)
-        k4, _ = kernel_time(r4["engine"], args.kernel_steps)
-        r4["engine"].close()
-        value, ms, npts, steps = r4["throughput"], r4["ms_per_step"], r4["npts"], args.steps
-        n_c = [15 if any(a is not None for a in (b.case.aniso or [])) else 9 for b in built]
-        B = sum((36.0 + 4.0 * (nc + 1)) * b.nglob() for nc, b in zip(n_c, built)) / npts
-        achieved = B * npts / (k4 * 1e-3) / 1e9
-        traffic = None
+        c4 = run_c4(args.steps, args.warmup, tile_z=args.tile_z, kernel_steps=args.kernel_steps)
+        value, ms, npts, steps = c4["throughput"], c4["ms_per_step"], c4["npts"], args.steps
+        achieved = c4["achieved_GBs"]
+        traffic = profiled_traffic("step_kernel_uneven96_c4.md")
         cfg = {"workload": "uneven batch: 96 instances, 200x100..1600x400 elements, 1-4 layers, iso or "
                            "hexagonal up to 6%, one persistent kernel per step", "instances": 96,
-               "nglob_total": int(npts), "kernel_ms": k4, "bytes_per_point_step_alg": B}
+               "nglob_total": c4["npts"], "kernel_ms": c4["kernel_ms"],
+               "bytes_per_point_step_alg": c4["bytes_per_point_step_alg"]}
         cpu_case = lambda: configs.c2_case(nst=10)
     elif args.config == "c5":
-        # the full serving configuration: 20/s for 300 s of C4-size requests
-        sv = run_serving(20.0, 300.0, (200, 1600), (100, 400), (2000, 12000))
+        # the serving configuration over a bounded arrival window (--serve-window;
+        # the stated 300 s of 20 requests/s offers several times what one GPU
+        # serves, see load_over_served): C4-size requests, 200 GB budget
+        sv = run_serving(20.0, args.serve_window, (200, 1600), (100, 400), (2000, 12000))
         value, ms, steps = sv["throughput"], float("nan"), 0
         achieved, traffic = float("nan"), None
-        cfg = {"workload": "serving: Poisson 20 requests/s for 300 s, C4-style instances, 200 GB HBM budget",
-               "serving": sv}
+        cfg = {"workload": "serving: Poisson 20 requests/s for %.0f s of arrivals, C4-style instances, 200 GB HBM "
+                           "budget" % args.serve_window, "serving": sv}
         cpu_case = lambda: configs.c2_case(nst=10)
     elif args.config == "c1b":
         # the literal reading: 64 waves on the C1 mesh (321,201 points each)
@@ -421,19 +554,25 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-        log("cpu baseline: %d threads, %d steps" % (threads, args.cpu_steps))
-        cb = cpu_baseline(cpu_case, args.cpu_steps, threads)
+        threads = cpu_threads()
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        log("cpu baseline: %d threads (OMP_PROC_BIND=%s), %d steps after %d warm-up steps" % (
+            threads, os.environ.get("OMP_PROC_BIND"), args.cpu_steps, args.cpu_warmup))
+        cb = cpu_baseline(cpu_case, args.cpu_steps, args.cpu_warmup)
+        env = "OMP_NUM_THREADS=%d (nproc %d, affinity %d), OMP_PROC_BIND=%s" % (
+            threads, os.cpu_count() or 0, len(os.sched_getaffinity(0)), os.environ.get("OMP_PROC_BIND"))
         cpu = {"value": cb["value"], "unit": "GLL point-steps/s", "cores": threads, "kind": "port",
-               "sample": "faithful FP32 port of sem2d_psvsh (oracle/sem_oracle.c, -O3, OpenMP on the node "
-                         "loops as the reference), 1 instance of the %d-point mesh, %d steps (%.1f s)"
-                         % (cb["nglob"], cb["steps"], cb["wall"])}
+               "sample": "faithful FP32 port of sem2d_psvsh (oracle/sem_oracle.c, -O3 -march=x86-64-v3, OpenMP "
+                         "on the node loops, element loop serial as in every reference variant), 1 instance of "
+                         "the %d-point mesh of the batch's model, %d steps timed after %d warm-up steps (%.1f s); %s"
+                         % (cb["nglob"], cb["steps"], cb["warmup"], cb["wall"], env)}
         log("  cpu %.3e pt-steps/s" % cb["value"])
         if args.cpu_steps_o0 > 0:
-            c0 = cpu_baseline(cpu_case, args.cpu_steps_o0, threads, kind="f32_O0")
+            c0 = cpu_baseline(cpu_case, args.cpu_steps_o0, args.cpu_warmup_o0, kind="f32_O0")
             cpu["makefile_flags"] = {"value": c0["value"], "cores": threads,
                                      "sample": "the same port built with the reference makefile's flags "
-                                               "(cc -fopenmp, no -O), %d steps (%.1f s)" % (c0["steps"], c0["wall"])}
+                                               "(cc -fopenmp, no -O), %d steps after %d warm-up steps (%.1f s)"
+                                               % (c0["steps"], c0["warmup"], c0["wall"])}
             log("  cpu, makefile flags: %.3e pt-steps/s" % c0["value"])
 
     line = {

@@ -165,10 +165,19 @@ void semsks_engine_destroy(semsks_engine *e);
    footprint does not fit the budget. */
 semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share_slot, int *slot);
 
-/* Device bytes one problem needs (for admission control) in an engine
-   that takes one step per launch; one with steps_per_sweep = 2 keeps a
+/* Device bytes one problem keeps while it is admitted (for admission
+   control) in an engine that takes one step per launch; one with steps_per_sweep = 2 keeps a
    fourth time level, 16 more bytes per (row-padded) node. */
 size_t semsks_footprint(const semsks_problem *p, int shared_material);
+
+/* Device bytes an engine's admission scratch needs to admit p: its
+   boundary records when they are generated on the device (device_records =
+   1, semsks_planewave), the FP64 setup temporaries and the interpolation
+   schedule, and the upload of host records.  The engine keeps the largest
+   scratch it has needed (admissions then need no allocation that waits for
+   running steps) and counts it in its HBM budget, on top of the slots'
+   semsks_footprint. */
+size_t semsks_admission_bytes(const semsks_problem *p, int device_records);
 
 /* Advances every active instance by up to nsteps time steps (the loop body
    of sem2d_psvsh.c:418-512: GRT_Interface4/3, Stiff_Matrix,
@@ -177,6 +186,13 @@ size_t semsks_footprint(const semsks_problem *p, int shared_material);
    1: returns when the steps are done; 2: as 1, but launched eagerly (no
    graph) with every step-kernel launch timed (semsks_last_timing). */
 semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync);
+
+/* Uploads the tile tables and captures the hipGraphs that a following
+   semsks_advance(e, nsteps, 0 or 1) will replay, without running a step,
+   so that call launches no capture (e.g. before a timed region).  The
+   graphs stay valid until the instance set changes.  No counterpart in the
+   reference. */
+semsks_status semsks_prepare(semsks_engine *e, int nsteps);
 
 /* Current time level of a slot (-1 if empty) and whether it finished. */
 semsks_status semsks_slot_status(semsks_engine *e, int slot, int *level, int *done);

@@ -40,6 +40,7 @@ def load(kind: str = "f64") -> C.CDLL:
     L.orc_update.argtypes = [vp, vp, vp, vp, vp]
     L.orc_boundary_series.argtypes = [vp, C.c_int, vp, vp]
     L.orc_run.argtypes = [vp, C.c_int, vp, vp]
+    L.orc_run_fast.argtypes = [vp, C.c_int, vp, vp]
     L.orc_main.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_char_p]
     L.orc_gll.argtypes = [C.c_int, vp, vp]
     L.orc_dlag.argtypes = [C.c_int, vp, vp]
@@ -47,6 +48,7 @@ def load(kind: str = "f64") -> C.CDLL:
     L.orc_interp_series.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_float, vp, vp]
     L.orc_guard.argtypes = [vp, C.c_int]
     L.orc_sponge.argtypes = [vp, vp]
+    L.orc_set_step_clock.argtypes = [vp, C.c_int]
     L.orc_taper.argtypes = [vp, vp]
     L.orc_read_raymodel.argtypes = [C.c_char_p, C.c_int, C.c_int, vp, vp, vp, vp, C.POINTER(C.c_int),
                                     C.POINTER(C.c_int)]
@@ -155,7 +157,21 @@ class Oracle:
             raise OracleError(st, self.L.orc_last_error().decode())
         return g, gv
 
-    def run(self, nsteps: int | None = None, want_field: bool = False):
+    def run_timed(self, nsteps: int):
+        """run(nsteps) that also returns the wall clock (s) at the end of
+        every step."""
+        t = np.zeros(nsteps, np.float64)
+        self.L.orc_set_step_clock(t.ctypes.data, nsteps)
+        try:
+            tr = self.run(nsteps)
+        finally:
+            self.L.orc_set_step_clock(None, 0)
+        return tr, t
+
+    def run(self, nsteps: int | None = None, want_field: bool = False, fast: bool = False):
+        """Traces [nsteps][3][ntrace] (and the final field).  fast: the
+        element force swept in parallel colours (orc_run_fast), the same
+        arithmetic summed in another order at shared nodes."""
         n = self.nst if nsteps is None else nsteps
         tr = np.zeros((n, 3, max(self.ntrace, 1)), np.float32)
         fu = np.zeros((self.nz, self.nx, 3), self.dt) if want_field else None
@@ -163,7 +179,8 @@ class Oracle:
         if self.cwd:
             os.chdir(self.cwd)
         try:
-            st = self.L.orc_run(self.h, n, tr.ctypes.data, None if fu is None else fu.ctypes.data)
+            run = self.L.orc_run_fast if fast else self.L.orc_run
+            st = run(self.h, n, tr.ctypes.data, None if fu is None else fu.ctypes.data)
         finally:
             os.chdir(old)
         if st:

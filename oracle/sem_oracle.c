@@ -39,6 +39,7 @@
 #include <string.h>
 #include <math.h>
 #include <stdint.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -344,6 +345,10 @@ typedef struct {
     real *gllp, *gllw, *dl, *dl_jei, *dl_gllw;
     real je, jei, jb;
     orc_cmat *mesh;
+    /* fast FP64 path (orc_run_fast): the distinct material records and a
+       record id per node, built on first use */
+    orc_cmat *rtab;
+    int *rid, nrtab;
     real *mass, *mdt2;
     orc_vec *ab;
     /* receivers */
@@ -633,6 +638,7 @@ static void absorb_matrix(orc_ctx *c)
 /* element force (Stiff_Matrix and callees)                                  */
 /* ------------------------------------------------------------------------ */
 typedef struct { real F0[3], F1[3], F2[3]; } orc_fik;
+#define SEMSKS_ORC_MAXNP 7
 
 /* Stress_ij (sem2d_psvsh.c:1345-1379); the material record is passed by value */
 static real stress_ij(int i, int j, real ukl[3][3], orc_cmat s)
@@ -703,6 +709,105 @@ static void stiff_matrix(const orc_ctx *c, orc_vec *S, const orc_vec *p)
                 }
         }
     free(F);
+}
+
+/* ------------------------------------------------------------------------ */
+/* K u for the fast FP64 path: the same element arithmetic as stiff_matrix,   */
+/* the elements swept in the four colours (ex%2, ez%2) -- no two elements of   */
+/* a colour share a node, so a colour's elements run in parallel without       */
+/* atomics -- and the material read through a per-node record id.  Only the   */
+/* order in which a shared node sums its element contributions differs from    */
+/* stiff_matrix (colour order instead of ex-outer, ez-inner).                  */
+/* ------------------------------------------------------------------------ */
+static real stress_p(int i, int j, real ukl[3][3], const orc_cmat *s)
+{
+    real st;
+    if (i == 1 && j == 1) return 0.;
+    st = s->c[i][j][0][0] * ukl[0][0];
+    st += s->c[i][j][0][1] * (ukl[0][1] + ukl[1][0]);
+    st += s->c[i][j][0][2] * (ukl[2][0] + ukl[0][2]);
+    st += s->c[i][j][1][1] * ukl[1][1];
+    st += s->c[i][j][1][2] * (ukl[1][2] + ukl[2][1]);
+    st += s->c[i][j][2][2] * ukl[2][2];
+    return st;
+}
+
+static void build_record_table(orc_ctx *c)
+{
+    size_t N = (size_t)c->nx * c->nz, q;
+    int cap = 16, k;
+    if (c->rid) return;
+    c->rid = (int *)xcalloc(N, sizeof(int));
+    c->rtab = (orc_cmat *)xcalloc(cap, sizeof(orc_cmat));
+    c->nrtab = 0;
+    for (q = 0; q < N; q++) {
+        int found = -1;
+        if (q > 0 && memcmp(&c->rtab[c->rid[q - 1]], &c->mesh[q], sizeof(orc_cmat)) == 0) found = c->rid[q - 1];
+        for (k = 0; found < 0 && k < c->nrtab; k++)
+            if (memcmp(&c->rtab[k], &c->mesh[q], sizeof(orc_cmat)) == 0) found = k;
+        if (found < 0) {
+            if (c->nrtab == cap) { cap *= 2; c->rtab = (orc_cmat *)realloc(c->rtab, cap * sizeof(orc_cmat)); }
+            c->rtab[c->nrtab] = c->mesh[q];
+            found = c->nrtab++;
+        }
+        c->rid[q] = found;
+    }
+}
+
+static void stiff_fast(const orc_ctx *c, orc_vec *S, const orc_vec *p)
+{
+    int n = c->hpn, np1 = n + 1, nx = c->nx, cx, cz;
+    long N = (long)c->nx * c->nz, q;
+#pragma omp parallel for schedule(static)
+    for (q = 0; q < N; q++) { S[q].u[0] = 0.; S[q].u[1] = 0.; S[q].u[2] = 0.; }
+    for (cx = 0; cx < 2; cx++)
+        for (cz = 0; cz < 2; cz++) {
+            int nexc = (c->nex - cx + 1) / 2, nezc = (c->nez - cz + 1) / 2;
+            long e;
+#pragma omp parallel for schedule(dynamic, 64)
+            for (e = 0; e < (long)nexc * nezc; e++) {
+                int ex = cx + 2 * (int)(e / nezc), ez = cz + 2 * (int)(e % nezc), al, be, i, th;
+                orc_fik F[SEMSKS_ORC_MAXNP * SEMSKS_ORC_MAXNP];
+                for (al = 0; al < np1; al++)
+                    for (be = 0; be < np1; be++) {
+                        size_t idx = (size_t)(ez * n + be) * nx + ex * n + al;
+                        const orc_cmat *mesh = &c->rtab[c->rid[idx]];
+                        real ukl[3][3], T[3][3];
+                        memset(ukl, 0, sizeof ukl);
+                        for (th = 0; th < np1; th++) {
+                            size_t k = (size_t)(ez * n + be) * nx + ex * n + th;
+                            real d = c->dl_jei[al * np1 + th];
+                            ukl[0][0] = ukl[0][0] + p[k].u[0] * d;
+                            ukl[1][0] = ukl[1][0] + p[k].u[1] * d;
+                            ukl[2][0] = ukl[2][0] + p[k].u[2] * d;
+                        }
+                        for (th = 0; th < np1; th++) {
+                            size_t k = (size_t)(ez * n + th) * nx + ex * n + al;
+                            real d = c->dl_jei[be * np1 + th];
+                            ukl[0][2] = ukl[0][2] + p[k].u[0] * d;
+                            ukl[1][2] = ukl[1][2] + p[k].u[1] * d;
+                            ukl[2][2] = ukl[2][2] + p[k].u[2] * d;
+                        }
+                        for (i = 0; i < 3; i++) {
+                            T[i][0] = stress_p(i, 0, ukl, mesh);
+                            T[i][2] = stress_p(i, 2, ukl, mesh);
+                            F[be * np1 + al].F0[i] = T[i][0] * c->jei;
+                            F[be * np1 + al].F2[i] = T[i][2] * c->jei;
+                        }
+                    }
+                for (al = 0; al < np1; al++)
+                    for (be = 0; be < np1; be++) {
+                        size_t idx = (size_t)(ez * n + be) * nx + ex * n + al;
+                        for (i = 0; i < 3; i++) {
+                            real it0 = 0., it2 = 0.;
+                            int a;
+                            for (a = 0; a < np1; a++) it0 = it0 + F[be * np1 + a].F0[i] * c->dl_gllw[a * np1 + al];
+                            for (a = 0; a < np1; a++) it2 = it2 + F[a * np1 + al].F2[i] * c->dl_gllw[a * np1 + be];
+                            S[idx].u[i] += (it0 * c->gllw[be] + it2 * c->gllw[al]) * c->je;
+                        }
+                    }
+            }
+        }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -875,7 +980,7 @@ ORC_EXPORT void orc_destroy(orc_ctx *c)
     if (!c) return;
     free(c->gllp); free(c->gllw); free(c->dl); free(c->dl_jei); free(c->dl_gllw);
     free(c->mesh); free(c->mass); free(c->mdt2); free(c->ab);
-    free(c->ix1); free(c->iz1); free(c->taper);
+    free(c->ix1); free(c->iz1); free(c->taper); free(c->rid); free(c->rtab);
     free(c);
 }
 
@@ -1169,7 +1274,36 @@ ORC_EXPORT int orc_read_raymodel(const char *path, int flat, int cap, float *cc,
 /* traces: [nsteps][3][ntrace] in the order u, w(sign-flipped), v, i.e. the  */
 /* record written to seis.u, seis.w, seis.v by kirrecord_zhl                 */
 /* ------------------------------------------------------------------------ */
+/* optional per-step wall clock of orc_run (the timed CPU baseline): when
+   set, step it of the next runs stores CLOCK_MONOTONIC seconds at its end
+   into t[it] for it < n */
+static double *orc_clock_t;
+static int orc_clock_n;
+ORC_EXPORT void orc_set_step_clock(double *t, int n) { orc_clock_t = t; orc_clock_n = t ? n : 0; }
+
+static double orc_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static int run_loop(orc_ctx *c, int nsteps, float *traces, real *final_u, int fast);
+
 ORC_EXPORT int orc_run(orc_ctx *c, int nsteps, float *traces, real *final_u)
+{
+    return run_loop(c, nsteps, traces, final_u, 0);
+}
+
+/* orc_run with the colour-parallel element force (stiff_fast): the FP64
+   oracle at sizes the reference-order sweep cannot reach in reasonable time */
+ORC_EXPORT int orc_run_fast(orc_ctx *c, int nsteps, float *traces, real *final_u)
+{
+    build_record_table(c);
+    return run_loop(c, nsteps, traces, final_u, 1);
+}
+
+static int run_loop(orc_ctx *c, int nsteps, float *traces, real *final_u, int fast)
 {
     size_t N = (size_t)c->nx * c->nz;
     orc_vec *uvw0 = (orc_vec *)xcalloc(N + 1, sizeof(orc_vec));
@@ -1206,7 +1340,8 @@ ORC_EXPORT int orc_run(orc_ctx *c, int nsteps, float *traces, real *final_u)
         } else {
             orc_vec *t;
             if (it < c->nt_sem) grt_interface(c, uvw, uvw0, g, g0, gv, gv0);
-            stiff_matrix(c, GS, uvw);
+            if (fast) stiff_fast(c, GS, uvw);
+            else stiff_matrix(c, GS, uvw);
             time_diff(c, uvw1, uvw, uvw0, GS);
             /* Time_diff_hybrid calls Absorbs unconditionally; with absorb=0
                the reference reads an unallocated Ab_matrix, here A = 0 */
@@ -1224,6 +1359,7 @@ ORC_EXPORT int orc_run(orc_ctx *c, int nsteps, float *traces, real *final_u)
                 tr[2 * c->ntrace + i] = (float)uvw[o].u[1];
             }
         }
+        if (it < orc_clock_n) orc_clock_t[it] = orc_now();
     }
     if (final_u) memcpy(final_u, uvw, N * 3 * sizeof(real));
     for (k = 0; k < nstream; k++) {

@@ -1892,12 +1892,55 @@ struct semsks_engine {
     bool tuning = false;               /* the autotuner is trying heights */
     bool tuned = false;
     std::map<uint64_t, int> tz_cache;   /* instance-set signature -> tile height */
-    /* device-resident plane-wave records waiting for the next admit */
+    /* admissions run on their own stream, so that their device work (FP64
+       setup, plane-wave records, boundary series, uploads) and the host
+       waits for it overlap a step round already queued on `stream`; their
+       temporaries live in one engine-owned scratch that only grows (no
+       hipFree, which would wait for the running round) */
+    hipStream_t setup = nullptr;
+    unsigned char *sbuf = nullptr;    /* admission scratch (counted in used) */
+    size_t scap = 0;
+    /* device-resident plane-wave records waiting for the next admit: the
+       first pend_bytes of the scratch */
     float *pend_psv = nullptr, *pend_sh = nullptr;
     int pend_nt = 0, pend_nx = 0, pend_nz = 0;
-    size_t pend_bytes = 0;            /* counted in used while they wait */
+    size_t pend_bytes = 0;
     void *scratch = nullptr;          /* this device's scratch array (semsks_stiff_dev) */
 };
+
+static semsks_status alloc_failed(hipError_t err, const char *what);
+
+static size_t scratch_growth(const semsks_engine *e, size_t bytes)
+{
+    return bytes > e->scap ? bytes - e->scap : 0;
+}
+
+/* makes the admission scratch at least `bytes` long, keeping its first
+   `keep` bytes (pending plane-wave records); it only grows, and growing
+   waits for the device (hipFree), which a serving loop meets only while
+   the largest request so far is admitted */
+static semsks_status setup_scratch(semsks_engine *e, size_t bytes, size_t keep)
+{
+    if (bytes <= e->scap) return SEMSKS_OK;
+    size_t cap = std::max(bytes, e->scap + e->scap / 2);
+    if (e->used - e->scap + cap > e->budget) cap = bytes;
+    if (e->used - e->scap + cap > e->budget)
+        return semsks_set_error(SEMSKS_ENOHBM, "admission scratch of %zu bytes does not fit the budget", cap);
+    unsigned char *nb = nullptr;
+    const hipError_t err = hipMalloc(&nb, cap);
+    if (err != hipSuccess) return alloc_failed(err, "admission scratch");
+    if (keep && e->sbuf) HIPCHK(hipMemcpyAsync(nb, e->sbuf, keep, hipMemcpyDeviceToDevice, e->setup));
+    HIPCHK(hipStreamSynchronize(e->setup));
+    if (e->sbuf) HIPCHK(hipFree(e->sbuf));
+    if (e->pend_psv) {
+        e->pend_sh = (float *)(nb + ((unsigned char *)e->pend_sh - e->sbuf));
+        e->pend_psv = (float *)nb;
+    }
+    e->used += cap - e->scap;
+    e->sbuf = nb;
+    e->scap = cap;
+    return SEMSKS_OK;
+}
 
 static int tile_rows(const semsks_engine *e, int nez)
 {
@@ -2171,6 +2214,7 @@ static semsks_status engine_init(semsks_engine *e)
     HIPCHK(hipMemGetInfo(&fr, &tot));
     e->budget = e->opts.hbm_budget ? e->opts.hbm_budget : (size_t)(0.9 * (double)fr);
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&e->setup, hipStreamNonBlocking));
     for (auto &v : e->ev) HIPCHK(hipEventCreate(&v));
     e->slots.resize(e->opts.max_slots);
     e->h_levels.assign(e->opts.max_slots, -1);
@@ -2219,12 +2263,13 @@ void semsks_engine_destroy(semsks_engine *e)
     if (e->stream) hipStreamSynchronize(e->stream);
     drop_graph(e);
     for (size_t k = 0; k < e->slots.size(); ++k) if (e->slots[k].used) free_slot(e, (int)k);
-    if (e->pend_psv) hipFree(e->pend_psv);
-    if (e->pend_sh) hipFree(e->pend_sh);
+    if (e->setup) hipStreamSynchronize(e->setup);
+    if (e->sbuf) hipFree(e->sbuf);
     hipFree(e->d_desc); hipFree(e->d_levels); hipFree(e->d_tiles); hipFree(e->d_ntiles); hipFree(e->d_qctr); hipFree(e->d_pctr);
     for (auto &v : e->ev) if (v) hipEventDestroy(v);
     for (auto &v : e->tev) hipEventDestroy(v);
     if (e->stream) hipStreamDestroy(e->stream);
+    if (e->setup) hipStreamDestroy(e->setup);
     delete e;
 }
 
@@ -2295,24 +2340,27 @@ static size_t admit_resident(const semsks_engine *e, const semsks_problem *p, bo
     return semsks_footprint(p, shared) + (e->opts.steps_per_sweep == 2 ? plane_of(p) * 16 : 0);
 }
 
-/* device bytes the admission of p holds only until semsks_admit returns:
-   the upload of host boundary records (none when they were generated on
-   the device, pending), the FP64 lumped mass and the small tables of the
-   one-off setup (none for a slot that shares its material), and the
-   interpolation schedule */
-static size_t admit_transient(const semsks_problem *p, bool shared, bool pending)
+/* bytes of admission temporaries after the pending records in the
+   scratch: descriptor, GLL weights and the Jacobian check (one 256-byte
+   line each), the FP64 lumped mass and the small tables of the one-off
+   setup (none for a slot that shares its material), the interpolation
+   schedule, and the upload of one stream's host boundary records (none
+   when they were generated on the device, pending) */
+static size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+static size_t admit_temp(const semsks_problem *p, bool shared, bool pending)
 {
-    const size_t mt = 2 * (size_t)(nx_of(p) + nz_of(p)), sht = (size_t)(nx_of(p) + nz_of(p));
-    size_t b = 0;
-    if (!pending && p->nt > 0) b += (size_t)p->nt * ((p->psv ? mt : 0) + (p->sh ? sht : 0)) * 4 + 512;
-    if (!shared) b += (size_t)nx_of(p) * nz_of(p) * 8 + 4096;
+    const size_t mt = 2 * (size_t)(nx_of(p) + nz_of(p));
     const int nlev = std::min(std::max(nt_sem_of(p), 1), std::max(p->nst, 1));
-    return b + (size_t)nlev * sizeof(SchedEntry) + 256;
+    size_t b = 3 * 256 + al256(sizeof(InstDesc));
+    if (!shared) b += al256((size_t)nx_of(p) * nz_of(p) * 8);
+    b += al256((size_t)nlev * sizeof(SchedEntry));
+    if (!pending && p->nt > 0 && (p->psv || p->sh)) b += al256((size_t)p->nt * mt * 4);
+    return b;
 }
 
 /* device bytes of the plane-wave records semsks_planewave(..., NULL, NULL)
    keeps for the next admission */
-static size_t pending_bytes(int nt, int nx, int nz) { return (size_t)nt * 3 * (nx + nz) * 4; }
+static size_t pending_bytes(int nt, int nx, int nz) { return al256((size_t)nt * 2 * (nx + nz) * 4) + al256((size_t)nt * (nx + nz) * 4); }
 
 /* an allocation failure maps to ENOHBM (the serving loop then waits for
    memory) when the device is out of memory, to EDEVICE otherwise; the
@@ -2324,13 +2372,20 @@ static semsks_status alloc_failed(hipError_t err, const char *what)
     return semsks_set_error(SEMSKS_EDEVICE, "%s: %s", what, hipGetErrorString(err));
 }
 
+size_t semsks_admission_bytes(const semsks_problem *p, int device_records)
+{
+    if (!p) return 0;
+    const size_t pend = device_records ? pending_bytes(p->nt, nx_of(p), nz_of(p)) : 0;
+    return pend + admit_temp(p, false, device_records != 0);
+}
+
 semsks_status semsks_room(semsks_engine *e, const semsks_problem *p, size_t pending)
 {
     if (!e || !p) return semsks_set_error(SEMSKS_EINVAL, "null argument");
     bool free_slot_left = false;
     for (auto &s : e->slots) free_slot_left = free_slot_left || !s.used;
     if (!free_slot_left) return semsks_set_error(SEMSKS_ENOSLOT, "all %zu slots busy", e->slots.size());
-    const size_t need = admit_resident(e, p, false) + admit_transient(p, false, pending > 0) + pending;
+    const size_t need = admit_resident(e, p, false) + scratch_growth(e, pending + admit_temp(p, false, pending > 0));
     if (e->used + need > e->budget)
         return semsks_set_error(SEMSKS_ENOHBM, "slot needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
     return SEMSKS_OK;
@@ -2381,7 +2436,8 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     }
     const bool pending = !p->psv && !p->sh && e->pend_psv && e->pend_nt == p->nt && e->pend_nx == nx_of(p) &&
                          e->pend_nz == nz_of(p);
-    const size_t need = admit_resident(e, p, share >= 0) + admit_transient(p, share >= 0, pending);
+    const size_t pendb = pending ? e->pend_bytes : 0;
+    const size_t need = admit_resident(e, p, share >= 0) + scratch_growth(e, pendb + admit_temp(p, share >= 0, pending));
     if (e->used + need > e->budget)
         return semsks_set_error(SEMSKS_ENOHBM, "slot needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
     for (int i = 0; i < p->ntrace; ++i)
@@ -2416,6 +2472,9 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     const int mt = 2 * (s.nx + s.nz), sht = s.nx + s.nz;
     semsks_status st = SEMSKS_OK;
 #define ACHK(x) do { const hipError_t ae_ = (x); if (ae_ != hipSuccess) { st = alloc_failed(ae_, #x); goto fail; } } while (0)
+    /* temporaries: the scratch after the pending records */
+    st = setup_scratch(e, pendb + admit_temp(p, share >= 0, pending), pendb);
+    if (st != SEMSKS_OK) { s = Slot(); return st; }
     {
         float *st_all = nullptr;
         /* three levels (four for an engine that may take two steps per
@@ -2424,7 +2483,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         d.nlev = e->opts.steps_per_sweep == 2 ? 4 : 3;
         const int nplanes = d.nlev * 4;
         ACHK(dalloc(s, &st_all, nodes * nplanes));
-        ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->stream));
+        ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->setup));
         for (int l = 0; l < d.nlev; ++l) {
             d.u[l] = st_all + 3 * l * nodes;
             d.comp[l] = reinterpret_cast<uint32_t *>(st_all + 3 * d.nlev * nodes + l * nodes);
@@ -2441,8 +2500,8 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         for (int i = 0; i < nh; ++i) tp[i] = (float)exp(alpha * i);
         float *dt = nullptr;
         ACHK(dalloc(s, &dt, (size_t)nh));
-        ACHK(hipMemcpyAsync(dt, tp.data(), nh * sizeof(float), hipMemcpyHostToDevice, e->stream));
-        ACHK(hipStreamSynchronize(e->stream));
+        ACHK(hipMemcpyAsync(dt, tp.data(), nh * sizeof(float), hipMemcpyHostToDevice, e->setup));
+        ACHK(hipStreamSynchronize(e->setup));
         d.taper = dt;
         d.atten = p->atten;
         d.nh = nh;
@@ -2469,10 +2528,10 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
             for (int iz = 0; iz < s.nz; ++iz)
                 for (int ix = 0; ix < s.nx; ++ix)
                     ids[node_at(s.bw, s.nz, ix, iz)] = p->mat_id[(size_t)iz * s.nx + ix];
-            ACHK(hipMemcpyAsync(mid, ids.data(), nodes * 2, hipMemcpyHostToDevice, e->stream));
-            ACHK(hipStreamSynchronize(e->stream));
+            ACHK(hipMemcpyAsync(mid, ids.data(), nodes * 2, hipMemcpyHostToDevice, e->setup));
+            ACHK(hipStreamSynchronize(e->setup));
         }
-        ACHK(hipMemcpyAsync(mat, p->mat_table, (size_t)p->n_mat * SEMSKS_MAT_STRIDE * 4, hipMemcpyHostToDevice, e->stream));
+        ACHK(hipMemcpyAsync(mat, p->mat_table, (size_t)p->n_mat * SEMSKS_MAT_STRIDE * 4, hipMemcpyHostToDevice, e->setup));
         {   /* element-uniform record ids */
             std::vector<uint16_t> em((size_t)p->nex * p->nez);
             const int n = p->hpn;
@@ -2484,38 +2543,35 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
                             if (p->mat_id[(size_t)(ez * n + b) * s.nx + ex * n + a] != r) { r = 0xFFFF; break; }
                     em[(size_t)ez * p->nex + ex] = r;
                 }
-            ACHK(hipMemcpyAsync(emat, em.data(), em.size() * 2, hipMemcpyHostToDevice, e->stream));
-            ACHK(hipStreamSynchronize(e->stream));
+            ACHK(hipMemcpyAsync(emat, em.data(), em.size() * 2, hipMemcpyHostToDevice, e->setup));
+            ACHK(hipStreamSynchronize(e->setup));
             s.emat = std::move(em);
         }
         d.mat_id = mid; d.emat = emat; d.mat = mat; d.mclass = mcl; d.edge_a = ea; d.edge_b = eb;
         /* FP64 setup on the device */
         {
-            InstDesc *dd = nullptr;
-            double *w64 = nullptr, *mass = nullptr;
-            unsigned long long *dev = nullptr;
-            ACHK(hipMalloc(&dd, sizeof(InstDesc)));
-            ACHK(hipMalloc(&w64, SEMSKS_MAXNP * sizeof(double)));
-            ACHK(hipMalloc(&mass, (size_t)s.nx * s.nz * sizeof(double)));
-            ACHK(hipMalloc(&dev, sizeof(unsigned long long)));
-            ACHK(hipMemsetAsync(dev, 0, sizeof(unsigned long long), e->stream));
-            ACHK(hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, e->stream));
-            hipLaunchKernelGGL(setup_tables_kernel, dim3(1), dim3(64), 0, e->stream, dd, w64, (double)p->eh);
-            hipLaunchKernelGGL(jacobian_kernel, dim3(256), dim3(256), 0, e->stream, p->nex, p->nez, p->hpn, (double)p->eh, dev);
+            unsigned char *tb = e->sbuf + pendb;
+            InstDesc *dd = (InstDesc *)tb;
+            double *w64 = (double *)(tb + al256(sizeof(InstDesc)));
+            unsigned long long *dev = (unsigned long long *)(tb + al256(sizeof(InstDesc)) + 256);
+            double *mass = (double *)(tb + al256(sizeof(InstDesc)) + 512);
+            ACHK(hipMemsetAsync(dev, 0, sizeof(unsigned long long), e->setup));
+            ACHK(hipMemcpyAsync(dd, &d, sizeof d, hipMemcpyHostToDevice, e->setup));
+            hipLaunchKernelGGL(setup_tables_kernel, dim3(1), dim3(64), 0, e->setup, dd, w64, (double)p->eh);
+            hipLaunchKernelGGL(jacobian_kernel, dim3(256), dim3(256), 0, e->setup, p->nex, p->nez, p->hpn, (double)p->eh, dev);
             const double eh = p->eh, je = (double)((double)eh * eh / 4.0), jb = eh / 2.0;
-            hipLaunchKernelGGL(mass_kernel, dim3(1024), dim3(256), 0, e->stream, d, w64, je, p->sdt, mass);
-            hipLaunchKernelGGL(mclass_kernel, dim3(64), dim3(256), 0, e->stream, mat, p->n_mat, p->hpn, w64, je, p->sdt, mcl);
-            hipLaunchKernelGGL(absorb_kernel, dim3(64), dim3(256), 0, e->stream, d, w64, jb, p->sdt, p->absorb, mass,
+            hipLaunchKernelGGL(mass_kernel, dim3(1024), dim3(256), 0, e->setup, d, w64, je, p->sdt, mass);
+            hipLaunchKernelGGL(mclass_kernel, dim3(64), dim3(256), 0, e->setup, mat, p->n_mat, p->hpn, w64, je, p->sdt, mcl);
+            hipLaunchKernelGGL(absorb_kernel, dim3(64), dim3(256), 0, e->setup, d, w64, jb, p->sdt, p->absorb, mass,
                                (double *)nullptr, ea, eb);
             InstDesc back;
-            ACHK(hipMemcpyAsync(&back, dd, sizeof back, hipMemcpyDeviceToHost, e->stream));
+            ACHK(hipMemcpyAsync(&back, dd, sizeof back, hipMemcpyDeviceToHost, e->setup));
             unsigned long long devbits = 0;
-            ACHK(hipMemcpyAsync(&devbits, dev, sizeof devbits, hipMemcpyDeviceToHost, e->stream));
-            ACHK(hipStreamSynchronize(e->stream));
+            ACHK(hipMemcpyAsync(&devbits, dev, sizeof devbits, hipMemcpyDeviceToHost, e->setup));
+            ACHK(hipStreamSynchronize(e->setup));
             memcpy(d.Dg, back.Dg, sizeof d.Dg); memcpy(d.Dw, back.Dw, sizeof d.Dw); memcpy(d.cw, back.cw, sizeof d.cw);
             double maxdev;
             memcpy(&maxdev, &devbits, sizeof maxdev);
-            hipFree(dd); hipFree(w64); hipFree(mass); hipFree(dev);
             if (!(maxdev <= 1.e-2)) { st = semsks_set_error(SEMSKS_EINVAL, "non-uniform Jacobian (max dev %g)", maxdev); goto fail; }
         }
     }
@@ -2536,18 +2592,18 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         s.d_fdesc = (InstDesc *)blk;
         s.d_fcount = (int *)(blk + dsz);
         s.d_ftiles = (int4 *)(blk + dsz + 256);
-        ACHK(hipMemcpyAsync(s.d_fdesc, &fd, sizeof fd, hipMemcpyHostToDevice, e->stream));
-        ACHK(hipMemcpyAsync(s.d_fcount, &nft, sizeof nft, hipMemcpyHostToDevice, e->stream));
-        if (nft) ACHK(hipMemcpyAsync(s.d_ftiles, ft.data(), ft.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
-        ACHK(hipStreamSynchronize(e->stream));
+        ACHK(hipMemcpyAsync(s.d_fdesc, &fd, sizeof fd, hipMemcpyHostToDevice, e->setup));
+        ACHK(hipMemcpyAsync(s.d_fcount, &nft, sizeof nft, hipMemcpyHostToDevice, e->setup));
+        if (nft) ACHK(hipMemcpyAsync(s.d_ftiles, ft.data(), ft.size() * sizeof(int4), hipMemcpyHostToDevice, e->setup));
+        ACHK(hipStreamSynchronize(e->setup));
     }
     /* boundary series */
     {
         float *G = nullptr, *Gv = nullptr;
         ACHK(dalloc(s, &G, (size_t)s.nlev * mt));
         ACHK(dalloc(s, &Gv, (size_t)s.nlev * sht));
-        ACHK(hipMemsetAsync(G, 0, (size_t)s.nlev * mt * 4, e->stream));
-        ACHK(hipMemsetAsync(Gv, 0, (size_t)s.nlev * sht * 4, e->stream));
+        ACHK(hipMemsetAsync(G, 0, (size_t)s.nlev * mt * 4, e->setup));
+        ACHK(hipMemsetAsync(Gv, 0, (size_t)s.nlev * sht * 4, e->setup));
         const float rdt = p->sdt / p->dp;
         const int nstream = p->solver == SEMSKS_SOLVER_PSVSH ? 2 : 1;
         int nrec[2] = {(p->psv || pending) ? p->nt : 0, (p->sh || pending) ? p->nt : 0};
@@ -2555,8 +2611,10 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         const int nlev_series = std::min(s.nt_sem, s.nlev);
         if (nlev_series > 0) {
             build_schedule(nlev_series, rdt, nstream, nrec, sch);
-            SchedEntry *dsch = nullptr;
-            ACHK(hipMalloc(&dsch, nlev_series * sizeof(SchedEntry)));
+            /* after the setup temporaries (the mass is no longer needed) */
+            unsigned char *tb = e->sbuf + pendb + 3 * 256 + al256(sizeof(InstDesc));
+            SchedEntry *dsch = (SchedEntry *)tb;
+            float *dup = (float *)(tb + al256((size_t)nlev_series * sizeof(SchedEntry)));
             for (int str = 0; str < nstream; ++str) {
                 const int width = str ? sht : mt;
                 const float *src = str ? p->sh : p->psv;
@@ -2564,23 +2622,21 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
                 bool own = false;
                 if (pending) drec = str ? e->pend_sh : e->pend_psv;
                 else if (src && p->nt > 0) {
-                    ACHK(hipMalloc(&drec, (size_t)p->nt * width * 4));
-                    ACHK(hipMemcpyAsync(drec, src, (size_t)p->nt * width * 4, hipMemcpyHostToDevice, e->stream));
+                    drec = dup;
+                    ACHK(hipMemcpyAsync(drec, src, (size_t)p->nt * width * 4, hipMemcpyHostToDevice, e->setup));
                     own = true;
                 }
                 if (!drec) continue;
-                ACHK(hipMemcpyAsync(dsch, sch[str].data(), nlev_series * sizeof(SchedEntry), hipMemcpyHostToDevice, e->stream));
-                hipLaunchKernelGGL(series_kernel, dim3(std::min(nlev_series, 65535)), dim3(256), 0, e->stream, drec,
+                ACHK(hipMemcpyAsync(dsch, sch[str].data(), nlev_series * sizeof(SchedEntry), hipMemcpyHostToDevice, e->setup));
+                hipLaunchKernelGGL(series_kernel, dim3(std::min(nlev_series, 65535)), dim3(256), 0, e->setup, drec,
                                    width, nrec[str], dsch, nlev_series, str ? Gv : G);
-                ACHK(hipStreamSynchronize(e->stream));
-                if (own) hipFree(drec);
+                ACHK(hipStreamSynchronize(e->setup));
+                (void)own;
             }
-            hipFree(dsch);
         }
-        if (pending) {
-            hipFree(e->pend_psv); hipFree(e->pend_sh);
-            e->pend_psv = e->pend_sh = nullptr; e->pend_nt = 0;
-            e->used -= e->pend_bytes;
+        if (pending) {              /* consumed: the scratch keeps its size */
+            e->pend_psv = e->pend_sh = nullptr;
+            e->pend_nt = 0;
             e->pend_bytes = 0;
         }
         d.G = G; d.Gv = Gv;
@@ -2593,8 +2649,8 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         for (int i = 0; i < p->ntrace; ++i) off[i] = (int)node_at(s.bw, s.nz, p->rec_ix[i], p->rec_iz[i]);
         ACHK(dalloc(s, &rec, off.size()));
         ACHK(dalloc(s, &tr, (size_t)p->nst * 3 * std::max(p->ntrace, 1)));
-        ACHK(hipMemcpyAsync(rec, off.data(), off.size() * 4, hipMemcpyHostToDevice, e->stream));
-        ACHK(hipMemsetAsync(tr, 0, (size_t)p->nst * 3 * std::max(p->ntrace, 1) * 4, e->stream));
+        ACHK(hipMemcpyAsync(rec, off.data(), off.size() * 4, hipMemcpyHostToDevice, e->setup));
+        ACHK(hipMemsetAsync(tr, 0, (size_t)p->nst * 3 * std::max(p->ntrace, 1) * 4, e->setup));
         d.rec = rec; d.traces = tr;
         /* receivers on interface nodes, recorded by post_kernel's thread for
            that node before it injects: per interface node the first such
@@ -2614,18 +2670,18 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         int *dhead = nullptr, *dnext = nullptr;
         ACHK(dalloc(s, &dhead, head.size()));
         ACHK(dalloc(s, &dnext, next.size()));
-        ACHK(hipMemcpyAsync(dhead, head.data(), head.size() * 4, hipMemcpyHostToDevice, e->stream));
-        ACHK(hipMemcpyAsync(dnext, next.data(), next.size() * 4, hipMemcpyHostToDevice, e->stream));
+        ACHK(hipMemcpyAsync(dhead, head.data(), head.size() * 4, hipMemcpyHostToDevice, e->setup));
+        ACHK(hipMemcpyAsync(dnext, next.data(), next.size() * 4, hipMemcpyHostToDevice, e->setup));
         d.inj_rec = dhead; d.rec_next = dnext;
     }
     /* level 0: its state, its record, then the injection of the first step */
-    hipLaunchKernelGGL(init_state_kernel, dim3(64), dim3(256), 0, e->stream, d);
-    if (p->ntrace > 0) hipLaunchKernelGGL(record0_kernel, dim3(1), dim3(256), 0, e->stream, d);
-    hipLaunchKernelGGL(inject0_kernel, dim3(16), dim3(256), 0, e->stream, d);
+    hipLaunchKernelGGL(init_state_kernel, dim3(64), dim3(256), 0, e->setup, d);
+    if (p->ntrace > 0) hipLaunchKernelGGL(record0_kernel, dim3(1), dim3(256), 0, e->setup, d);
+    hipLaunchKernelGGL(inject0_kernel, dim3(16), dim3(256), 0, e->setup, d);
     {
         int zero = 0;
-        ACHK(hipMemcpyAsync(e->d_levels + k, &zero, sizeof zero, hipMemcpyHostToDevice, e->stream));
-        ACHK(hipStreamSynchronize(e->stream));
+        ACHK(hipMemcpyAsync(e->d_levels + k, &zero, sizeof zero, hipMemcpyHostToDevice, e->setup));
+        ACHK(hipStreamSynchronize(e->setup));
     }
     if (hipGetLastError() != hipSuccess) { st = semsks_set_error(SEMSKS_EDEVICE, "setup kernel launch failed"); goto fail; }
 #undef ACHK
@@ -2636,7 +2692,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     *slot_out = k;
     return SEMSKS_OK;
 fail:
-    hipStreamSynchronize(e->stream);
+    hipStreamSynchronize(e->setup);
     for (void *q : s.allocs) hipFree(q);
     if (share >= 0) e->slots[share].refs--;
     s = Slot();
@@ -2777,6 +2833,28 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
         HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
         e->last_ms[0] = ms;
         e->last_ms[1] = kern_ms;
+    }
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_prepare(semsks_engine *e, int nsteps)
+{
+    if (!e || nsteps < 0) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
+    if (e->dirty) { semsks_status st = upload_tables(e); if (st) return st; }
+    if (!e->opts.use_graph) return SEMSKS_OK;
+    if (!graphs_current(e)) drop_graph(e);
+    /* the runs of 16, 8, 4, 2 and 1 steps semsks_advance would replay */
+    const int S = e->two ? 2 : 1;
+    for (int done = 0; done < nsteps;) {
+        const int rem = nsteps - done;
+        int i = 0;
+        while (i < 5 && ((16 >> i) > rem || (16 >> i) % S != 0)) ++i;
+        if (i == 5) break;
+        if (!e->gexec[i]) {
+            semsks_status st = build_graph(e, i);
+            if (st) return st;
+        }
+        done += 16 >> i;
     }
     return SEMSKS_OK;
 }
@@ -2980,36 +3058,35 @@ semsks_status semsks_planewave(semsks_engine *e, int nex, int nez, int hpn, int 
     if (!e || hpn < 3 || hpn > 6 || nex < 1 || nez < 1 || nt < 1 || !(p * beta >= 0 && p * beta < 1))
         return semsks_set_error(SEMSKS_EINVAL, "bad plane-wave arguments");
     const int nx = nex * hpn + 1, nz = nez * hpn + 1, mt = 2 * (nx + nz), sht = nx + nz;
-    float *dpsv = nullptr, *dsh = nullptr;
-    if (!psv && !sh) {            /* records kept for the next admission: counted in the budget */
-        if (e->pend_psv) { hipFree(e->pend_psv); e->pend_psv = nullptr; }
-        if (e->pend_sh) { hipFree(e->pend_sh); e->pend_sh = nullptr; }
-        e->used -= e->pend_bytes;
-        e->pend_bytes = 0;
-        e->pend_nt = 0;
-        const size_t need = pending_bytes(nt, nx, nz);
-        if (e->used + need > e->budget)
-            return semsks_set_error(SEMSKS_ENOHBM, "plane-wave records need %zu bytes, %zu of %zu in use", need,
-                                    e->used, e->budget);
-    }
-    hipError_t ae = hipMalloc(&dpsv, (size_t)nt * mt * 4);
-    if (ae != hipSuccess) return alloc_failed(ae, "plane-wave records");
-    ae = hipMalloc(&dsh, (size_t)nt * sht * 4);
-    if (ae != hipSuccess) { hipFree(dpsv); return alloc_failed(ae, "plane-wave records"); }
-    hipLaunchKernelGGL(planewave_kernel, dim3(2048), dim3(256), 0, e->stream, nex, nez, hpn, neh, (double)eh, nt,
-                       (double)dp, p, beta, f0, psi, dpsv, dsh);
-    HIPCHK(hipGetLastError());
-    if (psv || sh) {
-        if (psv) HIPCHK(hipMemcpyAsync(psv, dpsv, (size_t)nt * mt * 4, hipMemcpyDeviceToHost, e->stream));
-        if (sh) HIPCHK(hipMemcpyAsync(sh, dsh, (size_t)nt * sht * 4, hipMemcpyDeviceToHost, e->stream));
-        HIPCHK(hipStreamSynchronize(e->stream));
+    if (psv || sh) {              /* to host, through temporaries of their own */
+        float *dpsv = nullptr, *dsh = nullptr;
+        hipError_t ae = hipMalloc(&dpsv, (size_t)nt * mt * 4);
+        if (ae != hipSuccess) return alloc_failed(ae, "plane-wave records");
+        ae = hipMalloc(&dsh, (size_t)nt * sht * 4);
+        if (ae != hipSuccess) { hipFree(dpsv); return alloc_failed(ae, "plane-wave records"); }
+        hipLaunchKernelGGL(planewave_kernel, dim3(2048), dim3(256), 0, e->setup, nex, nez, hpn, neh, (double)eh, nt,
+                           (double)dp, p, beta, f0, psi, dpsv, dsh);
+        HIPCHK(hipGetLastError());
+        if (psv) HIPCHK(hipMemcpyAsync(psv, dpsv, (size_t)nt * mt * 4, hipMemcpyDeviceToHost, e->setup));
+        if (sh) HIPCHK(hipMemcpyAsync(sh, dsh, (size_t)nt * sht * 4, hipMemcpyDeviceToHost, e->setup));
+        HIPCHK(hipStreamSynchronize(e->setup));
         hipFree(dpsv); hipFree(dsh);
         return SEMSKS_OK;
     }
+    /* kept on the device for the next admission: the head of the admission
+       scratch (replacing records an earlier call kept) */
+    e->pend_psv = e->pend_sh = nullptr;
+    e->pend_nt = 0;
+    e->pend_bytes = 0;
+    const size_t need = pending_bytes(nt, nx, nz);
+    semsks_status st = setup_scratch(e, need, 0);
+    if (st != SEMSKS_OK) return st;
+    float *dpsv = (float *)e->sbuf, *dsh = (float *)(e->sbuf + al256((size_t)nt * mt * 4));
+    hipLaunchKernelGGL(planewave_kernel, dim3(2048), dim3(256), 0, e->setup, nex, nez, hpn, neh, (double)eh, nt,
+                       (double)dp, p, beta, f0, psi, dpsv, dsh);
+    HIPCHK(hipGetLastError());
     e->pend_psv = dpsv; e->pend_sh = dsh; e->pend_nt = nt; e->pend_nx = nx; e->pend_nz = nz;
-    e->pend_bytes = pending_bytes(nt, nx, nz);
-    e->used += e->pend_bytes;
-    HIPCHK(hipStreamSynchronize(e->stream));
+    e->pend_bytes = need;
     return SEMSKS_OK;
 }
 

@@ -8,7 +8,9 @@
  * (first come, first served), keeps every admitted instance's wavefield
  * resident, steps all of them together (one persistent launch per time
  * step), and retires each as it reaches its own last level, which frees its
- * memory for the requests still waiting.  Latency = completion - arrival.
+ * memory for the requests still waiting.  Each round of steps is queued on
+ * the device before the round's admissions run, so admission work overlaps
+ * stepping.  Latency = completion - arrival.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -76,7 +78,15 @@ semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int ste
     while (finished < n) {
         const double t = now_s() - t0;
         while (next < n && req[next].arrival_s <= t) queue[qt++] = next++;
-        /* admission, first come first served */
+        /* the next round of the admitted instances goes out first ... */
+        const int launched = nact > 0;
+        if (launched) {
+            st = semsks_advance(e, steps_per_round, 0);
+            if (st != SEMSKS_OK) break;
+            ++rounds;
+        }
+        /* ... and the admissions, first come first served, overlap it: their
+           host work, and their device work on the engine's admission stream */
         while (qh < qt) {
             semsks_request *r = &req[queue[qh]];
             int slot = -1;
@@ -101,13 +111,12 @@ semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int ste
             active[nact++] = queue[qh++];
             if (nact > max_active) max_active = nact;
         }
-        if (nact == 0) {
-            if (next < n) sleep_s(req[next].arrival_s - (now_s() - t0));
+        if (!launched) {
+            if (nact == 0 && next < n) sleep_s(req[next].arrival_s - (now_s() - t0));
             continue;
         }
-        st = semsks_advance(e, steps_per_round, 1);
+        st = semsks_sync(e);              /* the round is done */
         if (st != SEMSKS_OK) break;
-        ++rounds;
         /* harvest the instances that reached their last level */
         for (int i = 0; i < nact;) {
             semsks_request *r = &req[active[i]];

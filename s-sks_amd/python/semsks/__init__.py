@@ -27,7 +27,7 @@ STATUS = {0: "OK", 1: "EINVAL", 2: "EIO", 3: "ENOHBM", 4: "ENONFINITE", 5: "ERAN
 EXPORTS = [
     "semsks_load_par", "semsks_model_problem", "semsks_model_dims", "semsks_model_outdir",
     "semsks_model_free", "semsks_write_seis", "semsks_last_error", "semsks_engine_create",
-    "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_advance",
+    "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_admission_bytes", "semsks_advance", "semsks_prepare",
     "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_retire", "semsks_sync",
     "semsks_last_timing", "semsks_last_timed_steps", "semsks_sweep_steps", "semsks_tile_rows", "semsks_run_problem", "semsks_stiff", "semsks_stiff_dev",
     "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve", "semsks_uvw2sac",
@@ -94,7 +94,10 @@ def lib() -> C.CDLL:
     L.semsks_admit.argtypes = [vp, C.POINTER(Problem), C.c_int, C.POINTER(C.c_int)]
     L.semsks_footprint.restype = C.c_size_t
     L.semsks_footprint.argtypes = [C.POINTER(Problem), C.c_int]
+    L.semsks_admission_bytes.restype = C.c_size_t
+    L.semsks_admission_bytes.argtypes = [C.POINTER(Problem), C.c_int]
     L.semsks_advance.argtypes = [vp, C.c_int, C.c_int]
+    L.semsks_prepare.argtypes = [vp, C.c_int]
     L.semsks_slot_status.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.semsks_get_traces.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp]
     L.semsks_get_field.argtypes = [vp, C.c_int, vp]
@@ -193,6 +196,11 @@ def footprint(p: Problem, shared: bool = False) -> int:
     return int(lib().semsks_footprint(C.byref(p), int(shared)))
 
 
+def admission_bytes(p: Problem, device_records: bool = False) -> int:
+    """Bytes the engine's admission scratch needs for p (semsks_admission_bytes)."""
+    return int(lib().semsks_admission_bytes(C.byref(p), int(device_records)))
+
+
 class Engine:
     def __init__(self, max_slots=1, hbm_budget=0, tile_z=0, waves_per_cu=0, use_graph=1, steps_per_sweep=0):
         o = EngineOpts(hbm_budget=hbm_budget, max_slots=max_slots, tile_z=tile_z,
@@ -210,6 +218,10 @@ class Engine:
 
     def advance(self, n: int, sync: bool = True):
         _check(lib().semsks_advance(self._h, n, int(sync)))
+
+    def prepare(self, n: int):
+        """Captures the graphs advance(n) will replay (semsks_prepare)."""
+        _check(lib().semsks_prepare(self._h, n))
 
     def status(self, slot: int):
         lv, dn = C.c_int(), C.c_int()

@@ -140,6 +140,15 @@ def c5_requests(rate: float = 20.0, duration: float = 300.0, seed: int = SEED, n
     return list(zip(out, cases))
 
 
+def c5_sample(n: int, seed: int = SEED, frac: float = 0.05):
+    """The seeded sample of C5 requests whose FP32 traces are checked against
+    the FP64 oracle: request i is in it when the (seed + 55) stream's i-th
+    uniform draw is below frac.  The draw of a request does not depend on
+    how many requests the window holds."""
+    u = np.random.default_rng(seed + 55).random(max(n, 1))
+    return [i for i in range(n) if u[i] < frac]
+
+
 def device_problem(model: Model, nt: int) -> Problem:
     """Copy of a loaded problem whose boundary records come from the next
     semsks_planewave(..., to_host=False) call on the engine."""

@@ -248,7 +248,8 @@ def test_serving_waits_for_memory_and_rejects_what_never_fits():
     probs = [b.problem(b.case.nst + 3) for b in built]
     pws = [b.planewave_args(p.nt)[7:11] for b, p in zip(built, probs)]
     small_fp = [semsks.footprint(p) for k, p in enumerate(probs) if k != 3]
-    budget = 2 * max(small_fp) + min(small_fp) // 2
+    scratch = max(semsks.admission_bytes(p, True) for k, p in enumerate(probs) if k != 3)
+    budget = 2 * max(small_fp) + min(small_fp) // 2 + scratch     # the engine's admission scratch on top
     assert semsks.footprint(probs[3]) > budget
     arrivals = [0.0, 0.0, 0.0, 0.001, 0.002]
     with semsks.Engine(max_slots=8, hbm_budget=budget) as e:

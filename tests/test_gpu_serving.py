@@ -1,0 +1,68 @@
+"""Serving at the stated request sizes over a bounded window: the requests of
+the first 3 s of the seeded C5 stream (Poisson, 20 requests/s; meshes of
+200x100 .. 1600x400 elements, 1-4 layers, iso or tilted hexagonal, 2000 ..
+12000 steps) through the continuous-batching loop (semsks_serve) against a
+200 GB budget.  Every request completes with finite traces; the seeded 5%
+sample (configs.c5_sample) matches the FP64 oracle's seismograms of the same
+requests (tests/golden/c5_sample_fp64_traces.npz, written by
+tests/golden/make_c5_sample_fp64.py) within 1e-4 relative L2 and peak
+misfit, per component; and the two smallest sampled requests give the same
+traces, bit for bit, when run alone.  The reference has no serving loop: it
+runs each request as its own process (applications/test/prem_epi83.pl:81-82)."""
+import os
+
+import numpy as np
+import pytest
+
+import semsks
+from semsks import configs
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_c5_window_served_and_sample_matches_fp64():
+    path = os.path.join(GOLD, "c5_sample_fp64_traces.npz")
+    if not os.path.exists(path):
+        pytest.skip("the FP64 fixture of the C5 sample is not written (tests/golden/make_c5_sample_fp64.py)")
+    ref = np.load(path)
+    window = float(ref["window_s"])
+    dec = int(ref["decim"])
+    reqs = configs.c5_requests(duration=window)
+    sample = configs.c5_sample(len(reqs))
+    assert sample == [int(i) for i in ref["indices"]]
+    print("case: C5 serving, %d requests in %.0f s, sample %s" % (len(reqs), window, sample), flush=True)
+    built = [configs.Built(c) for _, c in reqs]
+    probs, pws = [], []
+    for b in built:
+        pr = b.problem(b.case.nst + 3)
+        probs.append(pr)
+        a = b.planewave_args(pr.nt)
+        pws.append((a[7], a[8], a[9], a[10]))
+    with semsks.Engine(max_slots=1024, hbm_budget=int(200e9)) as e:
+        res, st = e.serve(probs, [a for a, _ in reqs], pws, steps_per_round=16, want_traces=True)
+    print("served in %.1f s, max %d active" % (st["wall_s"], st["max_active"]), flush=True)
+    assert st["completed"] == len(reqs)
+    for r in res:
+        assert r["status"] == 0 and np.isfinite(r["traces"]).all()
+    worst = {}
+    for i in sample:
+        tr = res[i]["traces"][::dec]
+        for k, key in ((0, "u"), (1, "mw"), (2, "v")):
+            a, b = tr[:, k].astype(np.float64), ref["r%d_%s" % (i, key)].astype(np.float64)
+            assert a.shape == b.shape
+            l2 = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+            pk = np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)
+            worst[(i, key)] = (float(l2.max()), float(pk.max()))
+            assert np.abs(b).max() > 0.05
+            assert l2.max() <= 1e-4 and pk.max() <= 1e-4, (i, key, l2.max(), pk.max())
+    print("worst rel-L2, peak misfit per sampled request:", worst, flush=True)
+    small = sorted(sample, key=lambda i: built[i].nglob() * built[i].case.nst)[:2]
+    for i in small:
+        with semsks.Engine() as e:
+            e.planewave(*built[i].planewave_args(probs[i].nt), to_host=False)
+            s = e.admit(probs[i])
+            e.advance(probs[i].nst - 1)
+            solo = e.traces(s)
+        assert np.array_equal(solo.view(np.uint32), res[i]["traces"].view(np.uint32)), i

@@ -314,6 +314,26 @@ def test_energy_conserved_without_absorption(tmp_path):
     assert np.abs(E - E[0]).max() < 1e-10 * abs(E[0])
 
 
+def test_colour_parallel_fp64_path_equals_reference_order(tmp_path):
+    """orc_run_fast (the FP64 oracle of the large fixtures: the element force
+    swept in four colours in parallel) against orc_run (elements in the
+    reference's ex-outer, ez-inner order): the same arithmetic, so the final
+    fields agree to rounding of the shared-node sums (1e-12 of the peak) on
+    a case with a dipping interface, a tilted axis, the coupled solver and
+    absorbing edges."""
+    from semsks import aniso
+    c = inputs.small_case(nst=250, solver=0, psi_deg=30.0, sh_zero=False,
+                          interfaces=[([0.0, 50.0], [5.3, 9.1]), None, None],
+                          aniso=[aniso.hexagonal_tilted(6.5, 3.75, 2.9, 0.05, 30.0, 20.0), None, None])
+    d = os.fspath(tmp_path)
+    c.write(d)
+    o = pyoracle.Oracle(["par=run.par"], "f64", cwd=d)
+    _, fa = o.run(250, want_field=True)
+    _, fb = o.run(250, want_field=True, fast=True)
+    assert np.abs(fa).max() > 0.1
+    assert np.abs(fa - fb).max() <= 1e-12 * np.abs(fa).max()
+
+
 def test_faithful_fp32_tracks_fp64(tmp_path):
     c = inputs.small_case(nst=300, psi_deg=30.0, sh_zero=False)
     d = os.fspath(tmp_path)
