@@ -81,6 +81,43 @@ def layer_entry(C: np.ndarray):
     return d, dip6
 
 
+def bond_rotate_z(C: np.ndarray, angle_deg: float) -> np.ndarray:
+    """The matrix in a frame whose x1 axis is turned angle_deg counter-
+    clockwise about x3 from the old one: elastic_rotate's transformation
+    (semsyn/model/src/elastic_rotate.c:135-188, new-from-old direction
+    cosines cos(angle), cos(angle - 90), ...).  Turning the frame by +a
+    turns the medium by -a in it."""
+    return rotate(C, rotation(0.0, -angle_deg))
+
+
+def bond_dip(C: np.ndarray, angle_deg: float) -> np.ndarray:
+    """elastic_dip's transformation within the x1-x3 plane
+    (semsyn/model/src/elastic_dip.c:158-213): a vertical symmetry axis
+    tilts by angle_deg toward +x1."""
+    return rotate(C, rotation(angle_deg, 0.0))
+
+
+def thomsen(C: np.ndarray):
+    """(eps, gamma, delta) of a matrix with a vertical symmetry axis
+    (Thomsen 1986): eps = (c11 - c33) / (2 c33), gamma = (c66 - c44) / (2 c44),
+    delta = ((c13 + c44)^2 - (c33 - c44)^2) / (2 c33 (c33 - c44))."""
+    c11, c33, c13, c44, c66 = C[0, 0], C[2, 2], C[0, 2], C[3, 3], C[5, 5]
+    return ((c11 - c33) / (2 * c33), (c66 - c44) / (2 * c44),
+            ((c13 + c44) ** 2 - (c33 - c44) ** 2) / (2 * c33 * (c33 - c44)))
+
+
+# the semmodel layer line: 13 constants then rho, in the order the
+# reference's model builder writes them (new_sem, any_anisotropy.pl:114-143)
+# and discret reads them (sem2d_psvsh.c:700-731)
+SEMMODEL_ORDER = ("c11", "c12", "c13", "c16", "c22", "c23", "c26", "c33", "c36", "c44", "c45", "c55", "c66")
+
+
+def semmodel_line(C: np.ndarray, rho: float) -> list:
+    """[c11 c12 c13 c16 c22 c23 c26 c33 c36 c44 c45 c55 c66 rho] of a 6x6
+    matrix (1-based Voigt names; 4 = yz, 5 = xz, 6 = xy)."""
+    return [float(C[int(k[1]) - 1, int(k[2]) - 1]) for k in SEMMODEL_ORDER] + [float(rho)]
+
+
 def hexagonal_tilted(vp: float, vs: float, rho: float, strength: float, dip_deg: float, azimuth_deg: float,
                      delta: float = 0.0):
     """Thomsen eps = gamma = strength (the 'x% anisotropy' of the configs)."""

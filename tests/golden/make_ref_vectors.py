@@ -2,11 +2,13 @@
 
 Runs in the build container only (needs oracle/_ref/libref.so, built by
 oracle/Makefile from semsyn/src/interpl.c, readmodel.c and nrutil.c where
-they lie).  The vectors are data -- inputs and the reference's outputs --
+they lie, and oracle/_ref/elastic_rotate and elastic_dip, built from
+semsyn/model/src).  The vectors are data -- inputs and the reference's outputs --
 committed as ref_interpl.npz and ref_readmodel.npz.
 
   python tests/golden/make_ref_vectors.py          (all files)
   python tests/golden/make_ref_vectors.py mesh     (ref_interpl_mesh.npz only)
+  python tests/golden/make_ref_vectors.py elastic  (ref_elastic.npz only)
 """
 import ctypes as C
 import os
@@ -134,6 +136,43 @@ def mesh_vectors():
     print("wrote ref_interpl_mesh.npz", len(out), "arrays")
 
 
+def ref_elastic(prog, C, angle):
+    """Runs the reference's elastic_rotate / elastic_dip (oracle/_ref) on a
+    6x6 matrix: returns the rotated matrix it writes (%-8.2f)."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", prog)
+    with tempfile.TemporaryDirectory() as d:
+        src, dst = os.path.join(d, "in.ela"), os.path.join(d, "out.ela")
+        with open(src, "w") as f:
+            for row in C:
+                f.write(" ".join("%.4f" % v for v in row) + "\n")
+        subprocess.run([exe, src, dst, "%g" % angle], check=True, capture_output=True, cwd=d)
+        return np.loadtxt(dst).reshape(6, 6)
+
+
+def elastic_vectors():
+    """ref_elastic.npz: the model builder's Bond rotations about the vertical
+    axis (elastic_rotate) and within x1-x3 (elastic_dip) applied by the
+    reference's own programs to a transversely isotropic matrix with a
+    vertical axis and to the horizontal-axis matrix any_anisotropy.pl builds
+    (step 0, sem2d's 'stronger' factor 1.3, its asthenosphere vp/vs/rho)."""
+    vp, vs, rho = 8.1864, 4.5312, 3.4224
+    c11, c22, c44, c55 = vp * vp * rho * 1.3, vp * vp * rho, vs * vs * rho * 1.3, vs * vs * rho
+    hti = np.zeros((6, 6))
+    hti[0, 0], hti[1, 1], hti[2, 2] = c11, c22, c22
+    hti[0, 1] = hti[1, 0] = hti[0, 2] = hti[2, 0] = c11 - 2 * c44
+    hti[1, 2] = hti[2, 1] = c22 - 2 * c55
+    hti[3, 3], hti[4, 4], hti[5, 5] = c44, c55, c55
+    vti = np.array([[250, 80, 60, 0, 0, 0], [80, 250, 60, 0, 0, 0], [60, 60, 210, 0, 0, 0],
+                    [0, 0, 0, 60, 0, 0], [0, 0, 0, 0, 60, 0], [0, 0, 0, 0, 0, 85]], float)
+    out = {"vti": vti, "hti": hti, "angles": np.array([0.0, 30.0, 70.0, 135.0, -20.0])}
+    for name, C in (("vti", vti), ("hti", hti)):
+        for prog in ("elastic_rotate", "elastic_dip"):
+            out["%s_%s" % (name, prog)] = np.stack([ref_elastic(prog, C, a) for a in out["angles"]])
+    np.savez_compressed(os.path.join(HERE, "ref_elastic.npz"), **out)
+    print("wrote ref_elastic.npz")
+
+
 def main():
     rng = np.random.default_rng(20261203)
     out = {}
@@ -175,6 +214,9 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["mesh"]:
         mesh_vectors()
+    elif sys.argv[1:] == ["elastic"]:
+        elastic_vectors()
     else:
         main()
         mesh_vectors()
+        elastic_vectors()
