@@ -144,12 +144,11 @@ typedef struct semsks_engine_opts {
     int tile_z;            /* element rows per wave tile (0 = autotuned/default) */
     int waves_per_cu;      /* persistent grid size factor (0 = default) */
     int use_graph;         /* capture the step in a hipGraph (1) or not (0) */
-    int steps_per_sweep;   /* 0 or 1 (default): one time step per launch;
-                              2: two steps per sweep of the mesh when every
-                              instance allows it (hpn = 4, atten = 0, no
-                              receiver on the injection row/column), else one.
-                              Two steps read and write 35% fewer bytes but
-                              issue more vector instructions per point. */
+    int steps_per_sweep;   /* 0 or 1: one time step per launch (other
+                              values: SEMSKS_EINVAL; a sweep of two steps
+                              moved 35% fewer bytes but issued ~1.4x the
+                              vector instructions and was slower in every
+                              measured case, so it is not offered) */
 } semsks_engine_opts;
 
 semsks_status semsks_engine_create(const semsks_engine_opts *opts, semsks_engine **out);
@@ -166,8 +165,7 @@ void semsks_engine_destroy(semsks_engine *e);
 semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share_slot, int *slot);
 
 /* Device bytes one problem keeps while it is admitted (for admission
-   control) in an engine that takes one step per launch; one with steps_per_sweep = 2 keeps a
-   fourth time level, 16 more bytes per (row-padded) node. */
+   control). */
 size_t semsks_footprint(const semsks_problem *p, int shared_material);
 
 /* Device bytes an engine's admission scratch needs to admit p: its
@@ -221,9 +219,7 @@ semsks_status semsks_sync(semsks_engine *e);
 semsks_status semsks_last_timing(semsks_engine *e, double out[2]);
 /* Number of steps whose kernel time entered out[1] of semsks_last_timing. */
 int semsks_last_timed_steps(semsks_engine *e);
-/* Time steps one launch advances for the current instance set: 2 when
-   the two-step sweep is in use (semsks_engine_opts.steps_per_sweep = 2 and
-   every instance allows it), else 1.  Valid after semsks_advance. */
+/* Time steps one step-kernel launch advances: 1. */
 int semsks_sweep_steps(semsks_engine *e);
 /* Element rows per tile of the current tiling: the autotuned height once
    the autotuner has run for this instance set, the engine's default before,

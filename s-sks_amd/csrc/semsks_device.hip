@@ -34,26 +34,12 @@
  * node is written by exactly one lane, which applies the update in the same
  * pass while the row of u is still in registers.
  *
- * Two steps per sweep (step2_kernel, N = 4, engine option): the wave keeps
- * u(t+dt) of the rows it has just updated in registers and, one element row
- * behind, takes the second step from them, so a sweep reads levels it and
- * it-1 once and writes levels it+1 and it+2.  Its tiles own 9 of the 12
- * element columns and sweep two halo rows below and one above; u(t+dt) on
- * the interface row/column gets the next step's injection in registers.
- * It moves 35% fewer bytes per point-step but issues ~1.5x the vector
- * instructions (halo columns and rows, two live rows of state), and the
- * one-step kernel already keeps the vector ALUs ~2/3 busy, so one step per
- * launch is the default.
- *
  * State in HBM: three time levels of u rotating (level it in plane
  * it % 3), each an xyz plane (12 bytes per node) and a plane of 32-bit
  * compensation words that hold the packed Kahan compensations of that level
  * and of the level before.  A step reads the xyz of level it (with halos),
  * the xyz of level it-1 and the words of level it, and writes the xyz and
- * words of level it+1.  An engine that takes two steps per sweep keeps four
- * levels (it % 4) and also writes it+2, which then never overwrites a level
- * that a neighbouring tile may still read.  Rows are padded to a multiple
- * of 64 nodes.
+ * words of level it+1.  Planes are padded to a multiple of 64 nodes.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -950,8 +936,7 @@ __device__ __forceinline__ void row_force(const float (&u)[N + 1][3], float (&f)
 template <int N> struct RowFetch {
     float u[N][3];
     float un[3];
-    uint32_t w[N];     /* compensation words of level t: the owned nodes (every
-                          node of the row in a two-step sweep) */
+    uint32_t w[N];     /* compensation words of level t of the owned nodes */
     uint32_t wn;
     float pv[4][3];    /* per owned node: u(t-dt) */
     float m[4];
@@ -1297,352 +1282,6 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : 2) void step_kernel(const InstDes
     }
 }
 
-/* Two time steps per sweep (hpn = 4).  The tile owns element columns
-   x0..x0+8 and element rows z0..z0+rows-1; the wave holds element columns
-   x0-2..x0+9 (lane layout as process_tile).  First step (level it -> it+1)
-   on element rows z0-2..z0+rows: u(t+dt) is then valid on every node of
-   element columns x0-1..x0+8 and of element rows z0-1..z0+rows-1 (plus the
-   bottom node row of row z0+rows), which is all that the second step
-   (it+1 -> it+2), taken one element row behind on rows z0-1..z0+rows-1,
-   reads.  Each lane keeps its row of u(t+dt) and of u(t) in registers for
-   the second step; the top node row of an element comes from the lane that
-   holds it as the bottom row of the element above.  Before the second
-   step, u(t+dt) on the interface row/column gets the injection of level
-   it+1, exactly as post_kernel would apply it in place.  Arithmetic is
-   the same, node by node, as two one-step launches.  two = false: only the
-   first step (an instance with one step left). */
-constexpr int XC2 = 9;
-
-template <bool MIXED, bool ORTHO>
-__device__ void process_tile2(const InstDesc &D, int p, int it, bool two, int x0, int z0, int rows, int lane, float *lds)
-{
-    constexpr int N = 4;
-    using RL = RowLayout<N>;
-    constexpr int NP = RL::NP;
-    const int nex = D.nex, nez = D.nez, bw = D.bw, nx = D.nx, nz = D.nz;
-    const int e = lane / NP, b = lane - e * NP;
-    const bool act = lane < RL::NL;
-    const int ex = x0 - 2 + e;
-    const bool valid = act && ex >= 0 && ex < nex;
-    const bool own_x = valid && e >= 2 && e < 2 + XC2;
-    const bool last_x = ex == nex - 1;
-    const bool direct = valid && (e == RL::NE - 1 || last_x);
-    const bool kahan = D.kahan != 0;
-    const int ec = act ? e : 0;
-    float *Ub = lds + ec * RL::ESTR;
-    float *Qb = lds + RL::NE * RL::ESTR + ec * RL::ESTR;
-    float *CF1 = lds + 2 * RL::NE * RL::ESTR + ec * RL::CFS;
-    float *CF2 = lds + 2 * RL::NE * RL::ESTR + (RL::NE + ec) * RL::CFS;
-    const int exl = valid ? ex : 0;
-    const int ex_first = x0 >= 2 ? x0 - 2 : 0;
-    const RowCoef<N> rc = row_coef<N>(b);
-    /* storage columns of the lane's nodes 0..N-1 and of node N (the tiles of
-       a two-step sweep are narrower than a storage block, so elements of one
-       tile may lie in two blocks; an element never does) */
-    const uint32_t colb = (uint32_t)node_col(bw, nz, exl * N), colbn = (uint32_t)node_col(bw, nz, exl * N + N);
-    const int nbl = D.neh * N;
-    const bool inj = two && injects(D, it + 1);
-
-    const SEMSKS_GMEM float *U = gp(D.u[p]);
-    const SEMSKS_GMEM float *UP = gp(D.u[(p + 3) % 4]);
-    SEMSKS_GMEM float *U1 = gp(D.u[(p + 1) % 4]);
-    SEMSKS_GMEM float *U2 = gp(D.u[(p + 2) % 4]);
-    const SEMSKS_GMEM uint32_t *W = gp(D.comp[p]);
-    SEMSKS_GMEM uint32_t *W1 = gp(D.comp[(p + 1) % 4]);
-    SEMSKS_GMEM uint32_t *W2 = gp(D.comp[(p + 2) % 4]);
-
-    const int r0 = z0 >= 2 ? z0 - 2 : 0;            /* first row of step 1 */
-    const int rs2 = z0 >= 1 ? z0 - 1 : 0;           /* first row of step 2 */
-    const int rend = two ? z0 + rows + 1 : z0 + rows;
-    const int r1 = rend < nez ? rend : nez;         /* step 1 on rows [r0, r1) */
-
-    auto fetch = [&](int ez, RowFetch<N> &r) {
-        const int iz = ez * N + b;
-        const long rowoff = (long)(colb + (uint32_t)(iz * bw));
-        row_load<N>(U + 3 * rowoff, valid, r.u);
-        uint4 w4 = make_uint4(0u, 0u, 0u, 0u);
-        if (valid) w4 = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(W + rowoff);
-        r.w[0] = w4.x; r.w[1] = w4.y; r.w[2] = w4.z; r.w[3] = w4.w;
-        r.un[0] = r.un[1] = r.un[2] = 0.f;
-        r.wn = 0u;
-        if (direct) {
-            xyz_load(U, (long)(colbn + (uint32_t)(iz * bw)), r.un);
-            r.wn = W[(long)(colbn + (uint32_t)(iz * bw))];
-        }
-        r.rid = 0;
-        if constexpr (!MIXED) r.rid = (int)cp(D.emat)[(long)ez * nex + ex_first];
-    };
-    /* u(t-dt) and dt^2/M of the row, loaded at the top of its first step */
-    auto fetch_prev = [&](int ez, RowFetch<N> &r) {
-        const int iz = ez * N + b;
-        const long rowoff = (long)(colb + (uint32_t)(iz * bw));
-        const bool on = valid && (b < N || ez == nez - 1);
-        r.m[0] = r.m[1] = r.m[2] = r.m[3] = 0.f;
-        row_load<4>(UP + 3 * rowoff, on, r.pv);
-        if (on) {
-            const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
-            const bool inner = ex > 0 && iz != 0 && iz != nz - 1;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int mid = MIXED ? (int)idrow[a] : r.rid;
-                r.m[a] = inner ? gp(D.mclass)[(long)mid * 25 + b * 5 + a] : mdt2_of<N>(D, mid, ex * N + a, iz);
-            }
-        }
-    };
-
-    /* u(t+dt) of the lane's row: nodes 0..N-1 (and N on the last element
-       column) from u(t) = uu/wu, u(t-dt) = pv/pw, dt^2/M = m and the force;
-       lanes with no edge node take the interior form directly */
-    auto row_next = [&](int iz, const float (&f)[NP][3], const float (&uu)[NP][3], const uint32_t (&wu)[NP],
-                        const float (&pv)[NP][3], const float (&m)[4], float (&nu)[NP][3], uint32_t (&nw)[NP]) {
-        if (ex > 0 && iz != 0 && iz != nz - 1) {
-            const float one[3] = {1.f, 1.f, 1.f}, zero[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int a = 0; a < N; ++a) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) nu[a][k] = uu[a][k];
-                nw[a] = wu[a];
-                advance3(kahan, false, nu[a], nw[a], pv[a], f[a], m[a], one, zero);
-            }
-        } else {
-#pragma unroll
-            for (int a = 0; a < N; ++a)
-                next_node(D, kahan, edge_index(ex * N + a, iz, nx, nz, N), m[a], f[a], uu[a], wu[a], pv[a], nu[a],
-                          nw[a]);
-        }
-        if (last_x)
-            next_node(D, kahan, edge_index(nx - 1, iz, nx, nz, N), 0.f, f[N], uu[N], wu[N], pv[N], nu[N], nw[N]);
-    };
-
-    /* assembles a row force: z from the carried top row of the element
-       below (block CF), x from the element to the left */
-    auto assemble = [&](float (&f)[NP][3], float *CF) {
-        if (!valid) {
-#pragma unroll
-            for (int a = 0; a < NP; ++a)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) f[a][k] = 0.f;
-        }
-        if (b == 0) {
-            float cf[RL::CFS];
-#pragma unroll
-            for (int v = 0; v < (3 * NP + 3) / 4; ++v) {
-                const float4 t4 = *reinterpret_cast<const float4 *>(CF + 4 * v);
-                cf[4 * v] = t4.x; cf[4 * v + 1] = t4.y; cf[4 * v + 2] = t4.z; cf[4 * v + 3] = t4.w;
-            }
-#pragma unroll
-            for (int a = 0; a < NP; ++a)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) f[a][k] += cf[a * 3 + k];
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (act && b == N) {
-#pragma unroll
-            for (int a = 0; a < NP; ++a)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) CF[a * 3 + k] = f[a][k];
-        }
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const float s = __shfl(f[N][k], lane - NP, 64);
-            if (e > 0) f[0][k] += s;
-        }
-    };
-
-    auto row_material = [&](int ez, int rid, float (&f)[NP][3], const float (&u)[NP][3]) {
-        const long rowoff = (long)(colb + (uint32_t)((ez * N + b) * bw));
-        const SEMSKS_GMEM uint16_t *idrow = gp(D.mat_id) + rowoff;
-        const SEMSKS_GMEM uint16_t *idn = gp(D.mat_id) + (long)(colbn + (uint32_t)((ez * N + b) * bw));
-        if constexpr (!MIXED) {
-            const SEMSKS_CMEM float *r = cp(D.mat) + (long)rid * SEMSKS_MAT_STRIDE;
-            float C[15];
-#pragma unroll
-            for (int k = 0; k < 15; ++k) C[k] = r[1 + k];
-            row_force<N, true, ORTHO>(u, f, b, rc, Ub, Qb, C, gp(D.mat), idrow, idn, act);
-        } else {
-            row_force<N, false, ORTHO>(u, f, b, rc, Ub, Qb, nullptr, gp(D.mat), idrow, idn, act);
-        }
-    };
-
-    if (act && b == N) {
-#pragma unroll
-        for (int i = 0; i < RL::CFS; ++i) { CF1[i] = 0.f; CF2[i] = 0.f; }
-    }
-    __builtin_amdgcn_wave_barrier();
-
-    RowFetch<N> cur, prv;
-    float v1[NP][3], v1p[NP][3];
-    uint32_t w1[NP], w1p[NP];
-#pragma unroll
-    for (int a = 0; a < NP; ++a) {
-        w1[a] = w1p[a] = 0u;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) v1[a][k] = v1p[a][k] = 0.f;
-    }
-    fetch(r0, cur);
-    prv = cur;
-    for (int ez = r0; ez < rend; ++ez) {
-        RowFetch<N> nxt;
-        if (ez < r1) {
-            /* ---- step 1 on element row ez ---- */
-            const int iz = ez * N + b;
-            const bool top = ez == nez - 1;
-            float u[NP][3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-#pragma unroll
-                for (int a = 0; a < N; ++a) u[a][k] = cur.u[a][k];
-                const float nb = __shfl(cur.u[0][k], lane + NP, 64);
-                u[N][k] = direct ? cur.un[k] : nb;
-            }
-            float f[NP][3];
-#pragma unroll
-            for (int a = 0; a < NP; ++a)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) f[a][k] = 0.f;
-            fetch_prev(ez, cur);
-            row_material(ez, cur.rid, f, u);
-            assemble(f, CF1);
-            if (valid && (b < N || top)) {
-                float pv[NP][3];
-                uint32_t wu[NP];
-#pragma unroll
-                for (int a = 0; a < N; ++a) {
-                    pv[a][0] = cur.pv[a][0]; pv[a][1] = cur.pv[a][1]; pv[a][2] = cur.pv[a][2];
-                    wu[a] = cur.w[a];
-                }
-                wu[N] = cur.wn;
-                pv[N][0] = pv[N][1] = pv[N][2] = 0.f;
-                if (last_x) xyz_load(UP, node_at(D, nx - 1, iz), pv[N]);
-                row_next(iz, f, u, wu, pv, cur.m, v1, w1);
-                if (inj && (iz == nbl || (ex * N <= nbl && nbl <= ex * N + N))) {
-#pragma unroll
-                    for (int a = 0; a < NP; ++a) {
-                        const int ix = ex * N + a;
-                        if ((a < N || last_x) && is_inject_node(ix, iz, nbl)) {
-                            float d[3];
-                            inject_delta(D, it + 1, ix, iz, u[a][0], u[a][2], d);
-                            kahan3(kahan, v1[a], w1[a], d);
-                        }
-                    }
-                }
-                if (own_x && ez >= z0 && ez < z0 + rows) {
-                    const long q = (long)(colb + (uint32_t)(iz * bw)), qn = (long)(colbn + (uint32_t)(iz * bw));
-                    float o[4][3];
-#pragma unroll
-                    for (int a = 0; a < N; ++a)
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) o[a][k] = v1[a][k];
-                    row_store4(U1 + 3 * q, o);
-                    *reinterpret_cast<SEMSKS_GMEM uint4 *>(W1 + q) = make_uint4(w1[0], w1[1], w1[2], w1[3]);
-                    if (last_x) {
-                        xyz_store(U1, qn, v1[N]);
-                        W1[qn] = w1[N];
-                    }
-                }
-            }
-            /* the next row's loads go out before the second step */
-            __builtin_amdgcn_sched_barrier(0);
-            if (ez + 1 < r1) fetch(ez + 1, nxt);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        if (two && ez - 1 >= rs2) {
-            /* ---- step 2 on element row ez-1 ---- */
-            const int r = ez - 1;
-            const bool topr = r == nez - 1;
-            const bool mine = b < N || topr;          /* u(t+dt) of the lane's own row */
-            float u2[NP][3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-#pragma unroll
-                for (int a = 0; a < N; ++a) {
-                    const float s = __shfl(v1[a][k], lane - N, 64);
-                    u2[a][k] = mine ? v1p[a][k] : s;
-                }
-                const float sn = __shfl(v1[N][k], lane - N, 64);
-                const float dn = mine ? v1p[N][k] : sn;
-                const float nb = __shfl(u2[0][k], lane + NP, 64);
-                u2[N][k] = direct ? dn : nb;
-            }
-            float f[NP][3];
-#pragma unroll
-            for (int a = 0; a < NP; ++a)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) f[a][k] = 0.f;
-            row_material(r, prv.rid, f, u2);
-            assemble(f, CF2);
-            if (own_x && mine && r >= z0 && r < z0 + rows) {
-                const int iz = r * N + b;
-                float pv[NP][3], nu[NP][3];
-                uint32_t nw[NP];
-#pragma unroll
-                for (int a = 0; a < N; ++a) {
-                    pv[a][0] = prv.u[a][0]; pv[a][1] = prv.u[a][1]; pv[a][2] = prv.u[a][2];
-                }
-                pv[N][0] = prv.un[0]; pv[N][1] = prv.un[1]; pv[N][2] = prv.un[2];
-                row_next(iz, f, v1p, w1p, pv, prv.m, nu, nw);
-                const long q = (long)(colb + (uint32_t)(iz * bw)), qn = (long)(colbn + (uint32_t)(iz * bw));
-                float o[4][3];
-#pragma unroll
-                for (int a = 0; a < N; ++a)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) o[a][k] = nu[a][k];
-                row_store4(U2 + 3 * q, o);
-                *reinterpret_cast<SEMSKS_GMEM uint4 *>(W2 + q) = make_uint4(nw[0], nw[1], nw[2], nw[3]);
-                if (last_x) {
-                    xyz_store(U2, qn, nu[N]);
-                    W2[qn] = nw[N];
-                }
-            }
-        }
-        prv = cur;
-#pragma unroll
-        for (int a = 0; a < NP; ++a) {
-            w1p[a] = w1[a];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) v1p[a][k] = v1[a][k];
-        }
-        if (ez + 1 < r1) cur = nxt;
-    }
-}
-
-/* persistent two-step sweep over the tile list (hpn = 4 tiles only); an
-   instance with one step left takes only the first step; smax = 1 makes
-   every tile take one step */
-template <int DUMMY>
-__global__ __launch_bounds__(256, 2) void step2_kernel(const InstDesc *__restrict__ insts, const int *__restrict__ levels,
-                                                       const int4 *__restrict__ tiles, const int *__restrict__ ntiles, int smax)
-{
-    __shared__ float lds_blocks[4][RowLayout<4>::WAVE_FLOATS + RowLayout<4>::NE * RowLayout<4>::CFS];
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    const int nt = *ntiles;
-    for (int t = wave; t < nt; t += nwaves) {
-        int4 tile = tiles[t];
-        tile.x = __builtin_amdgcn_readfirstlane(tile.x);
-        tile.y = __builtin_amdgcn_readfirstlane(tile.y);
-        tile.z = __builtin_amdgcn_readfirstlane(tile.z);
-        tile.w = __builtin_amdgcn_readfirstlane(tile.w);
-        const InstDesc &D = insts[tile.x];
-        if (!D.active || D.hpn != 4) continue;
-        const int it = levels[tile.x];
-        if (it >= D.nst - 1) continue;
-        const bool two = smax >= 2 && it + 1 < D.nst - 1;
-        float *lds = lds_blocks[threadIdx.x >> 6];
-        const int rows = tile.w & 0xFFFF;
-        if (D.ortho) {
-            if (tile.w >> 16) process_tile2<true, true>(D, it % 4, it, two, tile.y, tile.z, rows, lane, lds);
-            else process_tile2<false, true>(D, it % 4, it, two, tile.y, tile.z, rows, lane, lds);
-        } else {
-            if (tile.w >> 16) process_tile2<true, false>(D, it % 4, it, two, tile.y, tile.z, rows, lane, lds);
-            else process_tile2<false, false>(D, it % 4, it, two, tile.y, tile.z, rows, lane, lds);
-        }
-    }
-}
-
-
 /* ========================================================================= */
 /* closed-form plane wave records (stand-in for the GRT producer)            */
 /* ========================================================================= */
@@ -1775,49 +1414,35 @@ struct Slot {
     InstDesc *d_fdesc = nullptr;
     int *d_fcount = nullptr;
     int4 *d_ftiles = nullptr;
-    bool two_ok = false;              /* may take two steps per sweep */
 };
 
 /* Tiling of one mesh.  A tile (slot, x0, z0, rows | mixed << 16) may run
    the single-record path when every element row it sweeps -- its own rows
-   and the halo rows (one below; two below and one above for a two-step
-   tile) -- holds one material record over the wave's element columns
-   (one halo column to the left, two for a two-step tile).  Each column of
-   tiles is cut along z so that rows that are not single-record (a layer
-   cut inside an element row) get short per-node-material tiles of their
-   own and the rest stay single-record tiles of up to tz rows. */
-struct TileGeom {
-    int xc;         /* owned element columns per tile */
-    int xhalo;      /* halo element columns to the left */
-    int ncols;      /* element columns a wave holds */
-    int lo, hi;     /* halo element rows below and above */
-};
-
-static TileGeom tile_geom(int n, bool two)
-{
-    if (two) return TileGeom{XC2, 2, 64 / 5, 2, 1};
-    return TileGeom{xchunk_of(n), 1, 64 / (n + 1), 1, 0};
-}
+   and the halo row below -- holds one material record over the wave's
+   element columns (one halo column to the left).  Each column of tiles is
+   cut along z so that rows that are not single-record (a layer cut inside
+   an element row) get short per-node-material tiles of their own and the
+   rest stay single-record tiles of up to tz rows. */
 
 /* walks the tiling of one mesh (owner slot o): emit(x0, z0, rows, mixed)
    for each x0 column in z order */
 template <class F>
-void walk_tiles(const Slot &o, int tz, bool two, F emit)
+void walk_tiles(const Slot &o, int tz, F emit)
 {
     const int nex = o.meta.nex, nez = o.meta.nez;
-    const TileGeom g = tile_geom(o.meta.hpn, two);
+    const int xc = xchunk_of(o.meta.hpn), ncols = 64 / (o.meta.hpn + 1);
     std::vector<char> single(nez);
-    for (int x0 = 0; x0 < nex; x0 += g.xc) {
-        const int ex0 = std::max(x0 - g.xhalo, 0), ex1 = std::min(nex, x0 - g.xhalo + g.ncols);
+    for (int x0 = 0; x0 < nex; x0 += xc) {
+        const int ex0 = std::max(x0 - 1, 0), ex1 = std::min(nex, x0 - 1 + ncols);
         for (int ez = 0; ez < nez; ++ez) {
             const uint16_t r = o.emat[(size_t)ez * nex + ex0];
             bool ok = r != 0xFFFF;
             for (int ex = ex0 + 1; ok && ex < ex1; ++ex) ok = o.emat[(size_t)ez * nex + ex] == r;
             single[ez] = ok;
         }
-        /* rows [z - lo, z + rows + hi) all single-record */
+        /* rows [z - 1, z + rows) all single-record */
         auto tile_single = [&](int z, int rows) {
-            for (int ez = std::max(z - g.lo, 0); ez < std::min(z + rows + g.hi, nez); ++ez)
+            for (int ez = std::max(z - 1, 0); ez < std::min(z + rows, nez); ++ez)
                 if (!single[ez]) return false;
             return true;
         };
@@ -1837,9 +1462,9 @@ void walk_tiles(const Slot &o, int tz, bool two, F emit)
     }
 }
 
-void build_tiles(const Slot &o, int slot, int tz, bool two, std::vector<int4> &t)
+void build_tiles(const Slot &o, int slot, int tz, std::vector<int4> &t)
 {
-    walk_tiles(o, tz, two, [&](int x0, int z0, int rows, int mixed) {
+    walk_tiles(o, tz, [&](int x0, int z0, int rows, int mixed) {
         t.push_back(make_int4(slot, x0, z0, rows | (mixed << 16)));
     });
 }
@@ -1878,9 +1503,6 @@ struct semsks_engine {
     hipGraphExec_t gexec[5] = {};
     unsigned hpn_mask = 0, graph_mask = 0;
     bool any_sponge = false, graph_sponge = false;
-    /* two steps per sweep (step2_kernel) for the current instance set, and
-       what the captured graph was built for */
-    bool two = false, graph_two = false;
     int graph_off[SEMSKS_MAXNP] = {0};  /* tile offsets the captured step was built with */
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> tev;      /* per-step timing events */
@@ -1975,17 +1597,6 @@ static semsks_status upload_tables(semsks_engine *e)
     HIPCHK(hipMemcpyAsync(e->d_desc, h.data(), h.size() * sizeof(InstDesc), hipMemcpyHostToDevice, e->stream));
     e->any_sponge = false;
     for (auto &s : e->slots) e->any_sponge |= s.used && s.d.atten > 0;
-    /* two steps per sweep when asked for and every instance allows it:
-       hpn = 4, no sponge, no receiver on an injection node (its trace of
-       the first level would see that level after the injection of the next
-       step) */
-    bool two = e->opts.steps_per_sweep == 2, any_used = false;
-    for (auto &s : e->slots)
-        if (s.used) { any_used = true; two = two && s.two_ok && s.meta.hpn == 4 && s.d.atten <= 0 && s.d.nlev == 4; }
-    if (e->two != (two && any_used)) {
-        e->two = two && any_used;
-        e->tuned = false;
-    }
     /* tiles, grouped by hpn, mixed (per-node material) tiles first since
        they take longest, then by z, slot and x: the waves of one XCD,
        which take consecutive tiles (see step_kernel), sweep neighbouring
@@ -2006,7 +1617,7 @@ static semsks_status upload_tables(semsks_engine *e)
             auto it = by_owner.find(ow);
             if (it == by_owner.end()) {
                 own.clear();
-                build_tiles(e->slots[ow], ow, tzg, e->two, own);
+                build_tiles(e->slots[ow], ow, tzg, own);
                 it = by_owner.emplace(ow, own).first;
             }
             for (int4 q : it->second) tn.push_back(make_int4((int)k, q.y, q.z, q.w));
@@ -2037,7 +1648,7 @@ static semsks_status upload_tables(semsks_engine *e)
     HIPCHK(hipStreamSynchronize(e->stream));
     e->dirty = false;
     if (e->opts.tile_z <= 0) {
-        auto it = e->tz_cache.find(tune_signature(e) | ((uint64_t)e->two << 32));
+        auto it = e->tz_cache.find(tune_signature(e));
         e->tuned = it != e->tz_cache.end();
         if (e->tuned && it->second != tzg) {
             e->tz_cur = it->second;
@@ -2077,19 +1688,6 @@ static int persistent_blocks(const semsks_engine *e, int n)
     return e->n_cu * per_cu;
 }
 
-/* the persistent grid of the two-step kernel */
-static int step2_blocks(const semsks_engine *e)
-{
-    static int per_cu = 0;
-    if (e->opts.waves_per_cu > 0) return std::max(1, e->n_cu * e->opts.waves_per_cu / 4);
-    if (per_cu < 1 &&
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&step2_kernel<0>), 256, 0) !=
-             hipSuccess ||
-         per_cu < 1))
-        per_cu = 2;
-    return e->n_cu * per_cu;
-}
-
 /* Tile height before (or without) the autotuner: the tallest of 32, 16,
    ..., 1 element rows that still gives every degree's launch at least two
    tiles per wave of its persistent grid.  Many short tiles keep the grid of
@@ -2100,9 +1698,8 @@ static int default_tile_rows(const semsks_engine *e)
 {
     int best = 32;
     for (int n = 3; n <= 6; ++n) {
-        const bool two = e->two && n == 4;
-        const long waves = 4L * (two ? step2_blocks(e) : persistent_blocks<false>(e, n));
-        const int xc = two ? XC2 : xchunk_of(n);
+        const long waves = 4L * persistent_blocks<false>(e, n);
+        const int xc = xchunk_of(n);
         bool any = false;
         int tz = 1;
         for (int c = 32; c >= 1; c /= 2) {
@@ -2138,25 +1735,17 @@ static void launch_tiles(const semsks_engine *e, int n, const InstDesc *desc, co
     }
 }
 
-/* one launch: steps every instance by up to smax levels (smax = 2 only in
-   two-step mode) */
-static semsks_status launch_step(semsks_engine *e, int smax, hipEvent_t ta, hipEvent_t tb)
+/* one launch: steps every instance by one level */
+static semsks_status launch_step(semsks_engine *e, hipEvent_t ta, hipEvent_t tb)
 {
     const int nslots = (int)e->slots.size();
-    if (!e->two) smax = 1;
     if (ta) HIPCHK(hipEventRecord(ta, e->stream));
-    if (e->two) {
-        if (e->hpn_mask & (1u << 4))
-            hipLaunchKernelGGL(step2_kernel<0>, dim3(step2_blocks(e)), dim3(256), 0, e->stream, e->d_desc, e->d_levels,
-                               e->d_tiles + e->tile_off[4], e->d_ntiles + 4, smax);
-    } else {
-        for (int n = 3; n <= 6; ++n)
-            if (e->hpn_mask & (1u << n))
-                launch_tiles<false>(e, n, e->d_desc, e->d_levels, e->d_tiles + e->tile_off[n], e->d_ntiles + n);
-    }
+    for (int n = 3; n <= 6; ++n)
+        if (e->hpn_mask & (1u << n))
+            launch_tiles<false>(e, n, e->d_desc, e->d_levels, e->d_tiles + e->tile_off[n], e->d_ntiles + n);
     if (tb) HIPCHK(hipEventRecord(tb, e->stream));
     if (e->any_sponge) hipLaunchKernelGGL(sponge_kernel, dim3(64, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels);
-    hipLaunchKernelGGL(post_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels, smax, e->d_qctr,
+    hipLaunchKernelGGL(post_kernel, dim3(16, nslots), dim3(256), 0, e->stream, e->d_desc, e->d_levels, 1, e->d_qctr,
                        e->d_pctr);
     HIPCHK(hipGetLastError());
     return SEMSKS_OK;
@@ -2175,7 +1764,7 @@ static void drop_graph(semsks_engine *e)
 static bool graphs_current(const semsks_engine *e)
 {
     return e->graph_mask == e->hpn_mask && memcmp(e->graph_off, e->tile_off, sizeof e->graph_off) == 0 &&
-           e->graph_sponge == e->any_sponge && e->graph_two == e->two;
+           e->graph_sponge == e->any_sponge;
 }
 
 /* captures graph[i], 16 >> i steps (a multiple of the steps per launch) */
@@ -2185,16 +1774,15 @@ static semsks_status build_graph(semsks_engine *e, int i)
     for (int j = 0; j < 5; ++j) any = any || e->gexec[j];
     if (any && !graphs_current(e)) drop_graph(e);
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    const int S = e->two ? 2 : 1, steps = 16 >> i;
-    for (int s = 0; s < steps; s += S) {
-        semsks_status st = launch_step(e, S, nullptr, nullptr);
+    const int steps = 16 >> i;
+    for (int s = 0; s < steps; ++s) {
+        semsks_status st = launch_step(e, nullptr, nullptr);
         if (st != SEMSKS_OK) { hipGraph_t g; hipStreamEndCapture(e->stream, &g); if (g) hipGraphDestroy(g); return st; }
     }
     HIPCHK(hipStreamEndCapture(e->stream, &e->graph[i]));
     HIPCHK(hipGraphInstantiate(&e->gexec[i], e->graph[i], nullptr, nullptr, 0));
     e->graph_mask = e->hpn_mask;
     e->graph_sponge = e->any_sponge;
-    e->graph_two = e->two;
     memcpy(e->graph_off, e->tile_off, sizeof e->graph_off);
     return SEMSKS_OK;
 }
@@ -2234,6 +1822,8 @@ semsks_status semsks_engine_create(const semsks_engine_opts *opts, semsks_engine
 {
     if (!out) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
     *out = nullptr;
+    if (opts && (opts->steps_per_sweep < 0 || opts->steps_per_sweep > 1))
+        return semsks_set_error(SEMSKS_EINVAL, "steps_per_sweep must be 0 or 1 (one time step per launch)");
     semsks_engine *e = new semsks_engine();
     if (opts) e->opts = *opts;
     if (e->opts.max_slots <= 0) e->opts.max_slots = 1;
@@ -2333,11 +1923,10 @@ static void build_schedule(int nlev, float rdt, int nstream, const int nrec[2], 
     }
 }
 
-/* device bytes a slot for p keeps: its footprint, plus the two extra time
-   levels of the two-step sweep */
-static size_t admit_resident(const semsks_engine *e, const semsks_problem *p, bool shared)
+/* device bytes a slot for p keeps */
+static size_t admit_resident(const semsks_engine *, const semsks_problem *p, bool shared)
 {
-    return semsks_footprint(p, shared) + (e->opts.steps_per_sweep == 2 ? plane_of(p) * 16 : 0);
+    return semsks_footprint(p, shared);
 }
 
 /* bytes of admission temporaries after the pending records in the
@@ -2454,11 +2043,6 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     s.nlev = std::max(1, std::min(s.nt_sem, p->nst));
     s.share = share;
     s.mat_hash = share >= 0 ? e->slots[share].mat_hash : material_hash(p);
-    s.two_ok = true;
-    for (int i = 0; i < p->ntrace; ++i)
-        if (((p->rec_iz[i] == p->neh * p->hpn && p->rec_ix[i] >= p->neh * p->hpn) ||
-             (p->rec_ix[i] == p->neh * p->hpn && p->rec_iz[i] > p->neh * p->hpn)))
-            s.two_ok = false;
     InstDesc &d = s.d;
     memset(&d, 0, sizeof d);
     d.nex = p->nex; d.nez = p->nez; d.nx = s.nx; d.nz = s.nz; d.bw = s.bw;
@@ -2477,10 +2061,10 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     if (st != SEMSKS_OK) { s = Slot(); return st; }
     {
         float *st_all = nullptr;
-        /* three levels (four for an engine that may take two steps per
-           sweep), each an xyz plane and a compensation-word plane; every
-           plane starts on a 256-byte boundary (nodes is a multiple of 64) */
-        d.nlev = e->opts.steps_per_sweep == 2 ? 4 : 3;
+        /* three levels, each an xyz plane and a compensation-word plane;
+           every plane starts on a 256-byte boundary (nodes is a multiple of
+           64) */
+        d.nlev = 3;
         const int nplanes = d.nlev * 4;
         ACHK(dalloc(s, &st_all, nodes * nplanes));
         ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->setup));
@@ -2579,7 +2163,7 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
        plane fits the scratch array) */
     if (share < 0 && scratch_bytes(nodes) <= SEMSKS_SCRATCH_BYTES) {
         std::vector<int4> ft;
-        build_tiles(s, 0, std::min(FORCE_TZ, std::max(1, p->nez)), false, ft);
+        build_tiles(s, 0, std::min(FORCE_TZ, std::max(1, p->nez)), ft);
         InstDesc fd = d;
         fd.active = 1;
         unsigned char *base = (unsigned char *)e->scratch;
@@ -2723,20 +2307,18 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     e->timed_steps = 0;
     int done = 0;
     /* autotuner: with tile_z unset, the first call that has the steps to
-       spare runs four real launches at each candidate tile height (six
-       for one step per launch, four for two-step sweeps) and keeps the
-       height whose fastest of the last three was fastest (the tiling does not
+       spare runs four real launches at each of the six candidate tile
+       heights and keeps the height whose fastest of the last three was
+       fastest (the tiling does not
        change the arithmetic, so these steps are ordinary steps of the run).
        Until then, and in calls too short for it (the serving loop's
        rounds), default_tile_rows picks the height.  Measured: one
        instance on the 200x100 mesh is fastest at 1 row, on the 2000x500
        mesh at 4 (many tiles per wave balance the persistent grid); a
        64-instance batch is nearly insensitive to the height */
-    const int S = e->two ? 2 : 1;     /* steps per launch */
-    static const int cand1[] = {1, 2, 4, 8, 16, 32}, cand2[] = {8, 16, 32, 64};
-    const int *cand = e->two ? cand2 : cand1;
-    const int ncand = e->two ? 4 : 6;
-    if (e->opts.tile_z <= 0 && !e->tuned && nsteps >= 4 * ncand * S && sync != 2) {
+    static const int cand[] = {1, 2, 4, 8, 16, 32};
+    const int ncand = 6;
+    if (e->opts.tile_z <= 0 && !e->tuned && nsteps >= 4 * ncand && sync != 2) {
         struct TuneFlag {
             semsks_engine *e;
             ~TuneFlag() { e->tuning = false; }
@@ -2755,7 +2337,7 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
             if (st) return st;
             /* four launches; the fastest of the last three counts */
             for (int r = 0; r < 4; ++r) {
-                st = launch_step(e, S, e->tev[2 * r], e->tev[2 * r + 1]);
+                st = launch_step(e, e->tev[2 * r], e->tev[2 * r + 1]);
                 if (st) return st;
             }
             HIPCHK(hipEventSynchronize(e->tev[7]));
@@ -2766,29 +2348,23 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
                 ms = std::min(ms, m);
             }
             if (ms < best_ms) { best_ms = ms; best = cand[c]; }
-            done += 4 * S;
+            done += 4;
         }
         e->tz_cur = best;
         semsks_status st = upload_tables(e);
         if (st) return st;
-        e->tz_cache[tune_signature(e) | ((uint64_t)e->two << 32)] = best;
+        e->tz_cache[tune_signature(e)] = best;
         e->tuned = true;
         drop_graph(e);
     }
     if (e->opts.use_graph && sync != 2) {
         /* runs of 16 captured steps, the rest in captured runs of 8, 4, 2
-           and 1 (a two-step sweep's odd last step eagerly) */
+           and 1 */
         if (!graphs_current(e)) drop_graph(e);
         while (done < nsteps) {
             const int rem = nsteps - done;
             int i = 0;
-            while (i < 5 && ((16 >> i) > rem || (16 >> i) % S != 0)) ++i;
-            if (i == 5) {
-                semsks_status st = launch_step(e, rem, nullptr, nullptr);
-                if (st) return st;
-                done += rem;
-                continue;
-            }
+            while ((16 >> i) > rem) ++i;
             if (!e->gexec[i]) {
                 semsks_status st = build_graph(e, i);
                 if (st) return st;
@@ -2799,8 +2375,8 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     } else {
         /* eager launches with an event pair around every step-kernel launch
            (the launches of the first 1024 steps of the call are timed) */
-        const int nl = (nsteps - done + S - 1) / S;
-        const int ntimed = sync ? std::min(nl, 1024 / S) : 0;
+        const int nl = nsteps - done;
+        const int ntimed = sync ? std::min(nl, 1024) : 0;
         while ((int)e->tev.size() < 2 * ntimed) {
             hipEvent_t v;
             HIPCHK(hipEventCreate(&v));
@@ -2808,11 +2384,10 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
         }
         int tsteps = 0;
         for (int j = 0; done < nsteps; ++j) {
-            const int s = std::min(S, nsteps - done);
-            semsks_status st = j < ntimed ? launch_step(e, s, e->tev[2 * j], e->tev[2 * j + 1]) : launch_step(e, s, nullptr, nullptr);
+            semsks_status st = j < ntimed ? launch_step(e, e->tev[2 * j], e->tev[2 * j + 1]) : launch_step(e, nullptr, nullptr);
             if (st) return st;
-            if (j < ntimed) tsteps += s;
-            done += s;
+            if (j < ntimed) ++tsteps;
+            ++done;
         }
         if (ntimed) {
             HIPCHK(hipEventSynchronize(e->tev[2 * ntimed - 1]));
@@ -2844,12 +2419,10 @@ semsks_status semsks_prepare(semsks_engine *e, int nsteps)
     if (!e->opts.use_graph) return SEMSKS_OK;
     if (!graphs_current(e)) drop_graph(e);
     /* the runs of 16, 8, 4, 2 and 1 steps semsks_advance would replay */
-    const int S = e->two ? 2 : 1;
     for (int done = 0; done < nsteps;) {
         const int rem = nsteps - done;
         int i = 0;
-        while (i < 5 && ((16 >> i) > rem || (16 >> i) % S != 0)) ++i;
-        if (i == 5) break;
+        while ((16 >> i) > rem) ++i;
         if (!e->gexec[i]) {
             semsks_status st = build_graph(e, i);
             if (st) return st;
@@ -2878,7 +2451,7 @@ semsks_status semsks_sync(semsks_engine *e)
 
 int semsks_sweep_steps(semsks_engine *e)
 {
-    return e && e->two ? 2 : 1;
+    return e ? 1 : 0;
 }
 
 int semsks_tile_rows(semsks_engine *e)
@@ -2955,7 +2528,7 @@ semsks_status semsks_stiff(semsks_engine *e, int k, const float *u, float *f)
     int4 *tl = nullptr;
     int *nt = nullptr;
     std::vector<int4> t;
-    build_tiles(s.share >= 0 ? e->slots[s.share] : s, 0, tile_rows(e, s.meta.nez), false, t);
+    build_tiles(s.share >= 0 ? e->slots[s.share] : s, 0, tile_rows(e, s.meta.nez), t);
     const int cnt = (int)t.size();
     HIPCHK(hipMalloc(&dd, sizeof d));
     sc.p[1] = dd;

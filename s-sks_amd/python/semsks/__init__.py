@@ -313,8 +313,7 @@ class Engine:
         return t[0], t[1], lib().semsks_last_timed_steps(self._h)
 
     def sweep_steps(self) -> int:
-        """Time steps per step-kernel launch for the current instances (2 in
-        the two-step sweep, else 1)."""
+        """Time steps per step-kernel launch (semsks_sweep_steps): 1."""
         return int(lib().semsks_sweep_steps(self._h))
 
     def tile_rows(self) -> int:

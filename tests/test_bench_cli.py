@@ -22,5 +22,6 @@ def test_more_than_one_gpu_is_refused():
 def test_help_lists_the_measurement_options():
     r = run("--help")
     assert r.returncode == 0
-    for opt in ("--steps", "--warmup", "--config", "--stream-ceiling", "--copy-calibration", "--steps-per-sweep"):
+    for opt in ("--steps", "--warmup", "--config", "--stream-ceiling", "--copy-calibration", "--serve-window",
+                "--cpu-warmup", "--no-c4"):
         assert opt in r.stdout, opt

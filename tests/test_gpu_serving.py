@@ -52,8 +52,12 @@ def test_c5_window_served_and_sample_matches_fp64():
         for k, key in ((0, "u"), (1, "mw"), (2, "v")):
             a, b = tr[:, k].astype(np.float64), ref["r%d_%s" % (i, key)].astype(np.float64)
             assert a.shape == b.shape
-            l2 = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
-            pk = np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)
+            # receivers the incident wave has not reached by the last step
+            # hold (near) zero: a floor of 1e-3 of the largest receiver's
+            # norm and peak keeps them from dividing by ~0
+            nb, pb = np.linalg.norm(b, axis=0), np.abs(b).max(axis=0)
+            l2 = np.linalg.norm(a - b, axis=0) / np.maximum(nb, 1e-3 * nb.max())
+            pk = np.abs(a - b).max(axis=0) / np.maximum(pb, 1e-3 * pb.max())
             worst[(i, key)] = (float(l2.max()), float(pk.max()))
             assert np.abs(b).max() > 0.05
             assert l2.max() <= 1e-4 and pk.max() <= 1e-4, (i, key, l2.max(), pk.max())
