@@ -460,6 +460,7 @@ def main():
                "tile_rows_batched": r3["tile_rows"], "tile_rows_plain": rp["tile_rows"],
                "c2_workload": "1 instance, 2000x500 mesh, tilted-axis hexagonal 4% upper mantle (15 constants)",
                "c2_value": r2["throughput"], "c2_ms_per_step": r2["ms_per_step"], "kernel_ms_c2": k2,
+               "c2_traffic_GBs": (profiled_traffic("step_kernel_plain_2000x500.md") or float("nan")) / (k2 * 1e-3) / 1e9,
                "c1_mesh_plain_value": r1p["throughput"], "c1_mesh_batched_value": r1b["throughput"],
                "c1_mesh_tile_rows_plain": r1p["tile_rows"], "c1_mesh_tile_rows_batched": r1b["tile_rows"],
                "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
@@ -470,7 +471,11 @@ def main():
             c4 = run_c4(args.steps, args.warmup, tile_z=args.tile_z, kernel_steps=args.kernel_steps)
             c4["traffic_bytes_per_launch"] = profiled_traffic("step_kernel_uneven96_c4.md")
             if c4["traffic_bytes_per_launch"]:
+                # the reference's bytes price per-node constants this build does not
+                # read (frac above 1): the measured traffic is the honest roofline
                 c4["traffic_GBs"] = c4["traffic_bytes_per_launch"] / (c4["kernel_ms"] * 1e-3) / 1e9
+                c4["frac_measured_traffic"] = c4["traffic_GBs"] / HBM_PEAK_GBS
+            c4["frac_alg_bytes"] = c4["achieved_GBs"] / HBM_PEAK_GBS
             c4["workload"] = ("uneven batch: 96 instances, 200x100..1600x400 elements, 1-4 layers, iso or tilted "
                               "hexagonal up to 6%, one persistent kernel per step")
             log("  c4 %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms" % (c4["throughput"], c4["ms_per_step"],
