@@ -34,6 +34,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -358,7 +359,8 @@ def main():
                     help="only run one device copy of this many MiB (PMC byte-counter calibration)")
     ap.add_argument("--stream-ceiling", type=int, default=0,
                     help="only time a device copy and a triad a = b + 2c over arrays of this many MiB "
-                         "(the practical HBM rate for 1:1 and 2:1 read:write traffic)")
+                         "(the practical HBM rate for 1:1 and 2:1 read:write traffic), and the step "
+                         "kernel's five-stream mix (s-sks_amd/build/semsks_stream_probe)")
     args = ap.parse_args()
     # the CPU baseline's OpenMP threads stay on neighbouring cores; set before
     # anything loads the OpenMP runtime
@@ -391,6 +393,11 @@ def main():
                 if r >= 2:
                     best = min(best, ev[0].elapsed_time(ev[1]))
             res[name] = {"bytes": nbytes, "best_ms": best, "GB_per_s": nbytes / best * 1e-6}
+        # the step kernel's own mix (28 B read, 16 B written per node, five
+        # streams) in hand-written grid-stride kernels, over 2^26 nodes
+        probe = os.path.join(ROOT, "s-sks_amd", "build", "semsks_stream_probe")
+        out = subprocess.run([probe], capture_output=True, text=True, check=True, timeout=120).stdout
+        res["hip_probe"] = json.loads(out)
         print(json.dumps(res))
         return
     if args.gpus != 1:

@@ -1230,7 +1230,8 @@ __device__ __forceinline__ void step_tile(const InstDesc *__restrict__ insts, co
    mixed tile takes the per-node material path.  The list is cut into NXCD
    contiguous runs, one per XCD, so that consecutive tiles -- the
    neighbours of one instance, which read the same 128-byte lines along
-   their shared edge -- run at the same time under the same L2.
+   their shared edges and stream one storage block -- run at the same time
+   under the same L2.
    Time steps with more tiles than waves (qctr != nullptr): each run is a
    queue with its counter in qctr[q * SEMSKS_QSTRIDE] (zero at launch:
    post_kernel clears them).  A wave takes the next tile of its own XCD's
@@ -1598,10 +1599,13 @@ static semsks_status upload_tables(semsks_engine *e)
     e->any_sponge = false;
     for (auto &s : e->slots) e->any_sponge |= s.used && s.d.atten > 0;
     /* tiles, grouped by hpn, mixed (per-node material) tiles first since
-       they take longest, then by z, slot and x: the waves of one XCD,
-       which take consecutive tiles (see step_kernel), sweep neighbouring
-       tiles of one instance side by side and share the lines along their
-       edges in L2 */
+       they take longest, then by slot, x and z: the waves of one XCD, which
+       take consecutive tiles (see step_kernel), sweep the tiles of one
+       column of one instance -- one storage block, contiguous in every
+       level plane -- and its neighbour columns at the same time, so that
+       the open DRAM pages and the lines along shared edges are few.  On the
+       64-wave 2000x500 batch this order ran the step kernel 2% faster than
+       z-major (z, slot, x) order, the single instance 5% faster */
     std::vector<int4> t;
     int cnt[SEMSKS_MAXNP] = {0};
     e->hpn_mask = 0;
@@ -1625,9 +1629,9 @@ static semsks_status upload_tables(semsks_engine *e)
         std::stable_sort(tn.begin(), tn.end(), [](const int4 &a, const int4 &b) {
             const int ma = a.w >> 16, mb = b.w >> 16;
             if (ma != mb) return ma > mb;
-            if (a.z != b.z) return a.z < b.z;
             if (a.x != b.x) return a.x < b.x;
-            return a.y < b.y;
+            if (a.y != b.y) return a.y < b.y;
+            return a.z < b.z;
         });
         t.insert(t.end(), tn.begin(), tn.end());
         cnt[n] = (int)t.size() - e->tile_off[n];

@@ -39,7 +39,15 @@ def main():
     reqs = configs.c5_requests(duration=WINDOW)
     sample = configs.c5_sample(len(reqs))
     out = {"window_s": np.float64(WINDOW), "decim": np.int32(DECIM), "indices": np.array(sample, np.int32)}
+    path = os.path.join(HERE, "c5_sample_fp64_traces.npz")
+    if os.path.exists(path):            # resume: keep the requests an earlier run finished
+        with np.load(path) as old:
+            if float(old["window_s"]) == WINDOW and int(old["decim"]) == DECIM and \
+                    list(old["indices"]) == list(sample):
+                out.update({k: old[k] for k in old.files if k.startswith("r")})
     for i in sample:
+        if "r%d_nst" % i in out:
+            continue
         t0 = time.time()
         case = reqs[i][1]
         with tempfile.TemporaryDirectory() as d:
@@ -53,7 +61,7 @@ def main():
         out["r%d_nst" % i] = np.int32(case.nst)
         print("request %d: %dx%d elements, %d steps, %.0f s" % (i, case.nex, case.nez, case.nst, time.time() - t0),
               flush=True)
-        np.savez_compressed(os.path.join(HERE, "c5_sample_fp64_traces.npz"), **out)
+        np.savez_compressed(path, **out)
     print("wrote", sorted(out))
 
 
