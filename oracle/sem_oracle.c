@@ -33,6 +33,38 @@
  *                             GLLWeight 256-278, GLL 323-366
  *   semsyn/src/readmodel.c    open_file 9-21, read_model 43-109, flatten 150-165
  *   semsyn/src/getpar/src/getpar.c  setpar/getpar/mstpar/endpar semantics
+ *
+ * How each part is pinned: checked against something it does not restate
+ * (tests/test_oracle_pins.py unless another file is named):
+ *   GLL, GLLWeight, dLag_N_i  closed forms (hpn 3-5), quadrature exactness,
+ *                             derivative-matrix identities
+ *   interpl, interp_const     bit-exact against the reference's own compiled
+ *                             interpl.c (tests/golden/ref_interpl*.npz)
+ *   the +-10 guard            its sequential rule traced by hand
+ *   read_model, flatten       bit-exact against the reference's own compiled
+ *                             readmodel.c (tests/golden/ref_readmodel.npz)
+ *   discret mesh sizes, receiver indices   the float-arithmetic cases
+ *   semmodel fill, Cijkl_3D, inv_index     Christoffel matrices of a
+ *                             closed-form transversely isotropic tensor
+ *                             (tests/test_material_pins.py)
+ *   layer cuts (lip)          the interface polylines' geometry
+ *                             (tests/test_material_pins.py)
+ *   atten_taper, atten_wave2  closed-form factors and strip membership
+ *                             (tests/test_material_pins.py)
+ *   Stiff_Matrix and callees  independent B-matrix assembly, null space,
+ *                             patch test
+ *   Mass_Matrix               sum of M = integral of rho
+ *   Time_diff_hybrid          discrete energy conserved without absorption
+ *   Absorb_Matrix, Absorbs    a normally incident plane pulse leaves through
+ *                             the Lysmer edge and returns from a free one
+ *   GRT_Interface3/4          closed-form amplitude radiated by the kick
+ *   stiff_fast (colour-parallel sweep)   within 1e-12 of Stiff_Matrix's order
+ *   REAL=float vs REAL=double tracking bound on a 300-step run
+ *   getpar semantics, status codes, greenfile layout   tests/test_formats.py
+ *   kirrecord_zhl             a vertically incident SV, SH or P pulse reaches
+ *                             the surface receivers in its own channel
+ *                             (u, -w, v) with the greenfile's sign and twice
+ *                             the radiated amplitude
  */
 #include <stdio.h>
 #include <stdlib.h>

@@ -9,7 +9,10 @@ Each oracle function is checked against something that does not restate it:
   * invariants: symmetry and null space of K, the patch test, mass = integral
     of rho, discrete energy conservation of the explicit update;
   * hand-traceable rules of the reference (interpolation delays at rdt=1,
-    the sequential guard, receiver indexing, the nez/eh float arithmetic).
+    the sequential guard, receiver indexing, the nez/eh float arithmetic);
+  * physics with closed forms: a plane pulse through a Lysmer or a free
+    edge, the amplitude the interface injection radiates, and a vertically
+    incident pulse recorded at the free surface.
 """
 import math
 import os
@@ -420,3 +423,53 @@ def test_injected_vertical_plane_wave_amplitude(tmp_path, solver, psi):
     assert abs(scale / expect - 1.0) < 0.01, (scale, expect)
     assert np.abs(col[sel] - scale * inc[sel]).max() < 0.03 * abs(scale)
     assert np.abs(f[1 - comp if comp < 2 else 0][:, o.nx // 2]).max() < 1e-9 * abs(scale)
+
+
+# ------------------------------------------------------- receiver records
+@pytest.mark.parametrize("kind,solver", [("SV", 0), ("SH", 0), ("P", 0), ("SV", 2)])
+def test_surface_record_of_a_vertical_pulse(tmp_path, kind, solver):
+    """kirrecord_zhl (sem2d_psvsh.c:2116-2165) writes u, -w and v; the
+    injection (GRT_Interface4, :2076-2105) adds minus the greenfile's w to
+    the solver's w.  So a vertically incident pulse put in one greenfile
+    channel (u for SV, w for P, the shgreenfile for SH) must come out in the
+    same seismogram channel with the greenfile's own sign, and at a free
+    surface with twice the amplitude the interface radiates (the impulse
+    w_0*eh/(2*v*dt) of GRT_Interface4, 1 for GRT_Interface3), arriving after
+    the travel time to the surface; the other two channels stay silent.  A
+    record with the sign of w not flipped, or two channels swapped, fails."""
+    vp, vs, rho, f0 = 6.0, 3.5, 2.7, 0.25
+    v = vp if kind == "P" else vs
+    c = inputs.Case(nex=80, eh=2.0, layers_thk=(6.0, 6.0, 6.002), vp=(vp,) * 3, vs=(vs,) * 3, rho=(rho,) * 3,
+                    jitter=0.0, p=0.0, f0=f0, psi_deg=90.0 if kind == "SH" else 0.0, sh_zero=(kind != "SH"),
+                    solver=solver, nrec=5, rec_x0=64.0, rec_x1=96.0, nst=3)
+    D = c.derived()
+    lead = 1.2 / f0
+    height = D["nez"] * D["eh"]
+    c.nst = int(round((lead + height / v + 1.2 / f0 + 1.0) / D["sdt"])) + 1
+    d = os.fspath(tmp_path)
+    c.write(d)
+    if kind == "P":     # u = 0, greenfile w = r(t - lead - z/vp) on the bottom row and the left column
+        nx, nz, nbl = D["nx"], D["nz"], c.neh * c.hpn
+        zs = inputs.node_coords(D["nez"], D["eh"], c.hpn)
+        sdt = float(np.float32(D["sdt"]))
+        psv = np.zeros((c.nst + 3, 2 * (nx + nz)), np.float32)
+        for it in range(c.nst + 3):
+            psv[it, nx:2 * nx] = inputs.ricker(it * sdt - lead - zs[nbl] / vp, f0)
+            psv[it, 2 * nx + nz:] = inputs.ricker(it * sdt - lead - zs / vp, f0)
+        psv.tofile(os.path.join(d, "greenfile"))
+    o = pyoracle.Oracle(["par=run.par"], "f64", cwd=d)
+    tr = o.run()
+    _, w, _ = pyoracle.gll(o.hpn, "f64")
+    scale = 1.0 if solver == 2 else w[0] * o.eh / (2.0 * v * o.sdt)
+    t = np.arange(tr.shape[0]) * o.sdt
+    top = inputs.node_coords(o.nez, o.eh, o.hpn)[-1]
+    expect = 2.0 * scale * inputs.ricker(t - lead - top / v, f0)
+    k = {"SV": 0, "P": 1, "SH": 2}[kind]
+    peak = np.abs(expect).max()
+    for r in range(tr.shape[2]):
+        fit = float(tr[:, k, r] @ expect / (expect @ expect))
+        assert abs(fit - 1.0) < 0.005, (r, fit)
+        assert np.abs(tr[:, k, r] - expect).max() < 0.04 * peak
+        for j in range(3):
+            if j != k:
+                assert np.abs(tr[:, j, r]).max() < 0.01 * peak, (r, j)
