@@ -2,8 +2,8 @@
  * semsks_kernels.h -- device data layout shared by the kernels and the engine.
  *
  * Layout in HBM, per instance (slot):
- *   u[l]      displacement of time level l (3 levels rotating, 4 for
- *             two-step sweeps): x, y, z per node, 12 bytes.  Nodes are
+ *   u[l]      displacement of time level l (3 levels rotating): x, y, z
+ *             per node, 12 bytes.  Nodes are
  *             stored in column blocks of bw node columns (the owned columns
  *             of one step-kernel tile, bw = (64/(hpn+1) - 1) * hpn), each
  *             block row after row: node (ix, iz) at
@@ -43,8 +43,8 @@
 struct InstDesc {
     int nex, nez, nx, nz, bw;       /* bw: node columns per storage block */
     int hpn, neh, nst, nt_sem, solver, kahan, ntrace, active, nlev;
-    /* displacement of nlev (3, or 4 for two-step sweeps) time levels,
-       level it in plane it % nlev: u[l] holds x, y, z per node (node q at
+    /* displacement of nlev = 3 time levels (entries 0-2 used), level it
+       in plane it % nlev: u[l] holds x, y, z per node (node q at
        u[l][3q..3q+2]), comp[l] one 32-bit word per node with the packed
        Kahan compensations of the three components at that level (bits
        0-14) and at the level before (bits 15-29) */
