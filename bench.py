@@ -396,6 +396,8 @@ def main():
         # the step kernel's own mix (28 B read, 16 B written per node, five
         # streams) in hand-written grid-stride kernels, over 2^26 nodes
         probe = os.path.join(ROOT, "s-sks_amd", "build", "semsks_stream_probe")
+        if not os.path.exists(probe):
+            sys.exit("%s is not built (make -C s-sks_amd)" % probe)
         out = subprocess.run([probe], capture_output=True, text=True, check=True, timeout=120).stdout
         res["hip_probe"] = json.loads(out)
         print(json.dumps(res))
