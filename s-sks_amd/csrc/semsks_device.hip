@@ -1423,12 +1423,20 @@ struct Slot {
    element columns (one halo column to the left).  Each column of tiles is
    cut along z so that rows that are not single-record (a layer cut inside
    an element row) get short per-node-material tiles of their own and the
-   rest stay single-record tiles of up to tz rows. */
+   rest stay single-record tiles of up to tz rows.  A run of R
+   single-record rows is cut into ceil(R / tz) tiles whose heights differ
+   by at most one row (100 rows at tz = 8: 9 tiles of 8 and 4 of 7, not 12
+   of 8 and a short one of 4), so no wave of the persistent grid is left
+   with a tile much shorter than the others. */
 
-/* walks the tiling of one mesh (owner slot o): emit(x0, z0, rows, mixed)
-   for each x0 column in z order */
-template <class F>
-void walk_tiles(const Slot &o, int tz, F emit)
+/* A run of rows of one tile column: single-record (every row from its
+   first, with the halo row below the first, holds one record) or
+   per-node-material. */
+struct ZRun { int x0, z, len, mixed; };
+
+/* the runs of every tile column of one mesh (owner slot o), in x then z
+   order; they do not depend on the tile height */
+void tile_runs(const Slot &o, std::vector<ZRun> &out)
 {
     const int nex = o.meta.nex, nez = o.meta.nez;
     const int xc = xchunk_of(o.meta.hpn), ncols = 64 / (o.meta.hpn + 1);
@@ -1441,26 +1449,46 @@ void walk_tiles(const Slot &o, int tz, F emit)
             for (int ex = ex0 + 1; ok && ex < ex1; ++ex) ok = o.emat[(size_t)ez * nex + ex] == r;
             single[ez] = ok;
         }
-        /* rows [z - 1, z + rows) all single-record */
-        auto tile_single = [&](int z, int rows) {
-            for (int ez = std::max(z - 1, 0); ez < std::min(z + rows, nez); ++ez)
-                if (!single[ez]) return false;
-            return true;
-        };
+        /* a single-record tile may start at z when rows z - 1 and z are */
+        auto starts = [&](int z) { return (z == 0 || single[z - 1]) && single[z]; };
         for (int z = 0; z < nez;) {
-            int r = 0;
-            while (r < tz && z + r < nez && tile_single(z, r + 1)) ++r;
-            if (r > 0) {
-                emit(x0, z, r, 0);
-                z += r;
-                continue;
+            int len = 0;
+            if (starts(z)) {
+                while (z + len < nez && single[z + len]) ++len;
+                out.push_back({x0, z, len, 0});
+            } else {
+                while (z + len < nez && !starts(z + len)) ++len;
+                out.push_back({x0, z, len, 1});
             }
-            r = 1;
-            while (r < tz && z + r < nez && !tile_single(z + r, 1)) ++r;
-            emit(x0, z, r, 1);
-            z += r;
+            z += len;
         }
     }
+}
+
+/* cuts runs into tiles of up to tz rows: emit(x0, z0, rows, mixed) */
+template <class F>
+void cut_runs(const std::vector<ZRun> &runs, int tz, F emit)
+{
+    for (const ZRun &r : runs) {
+        const int k = (r.len + tz - 1) / tz;
+        for (int i = 0, z = r.z; i < k; ++i) {
+            /* single-record runs in heights that differ by at most one
+               row; per-node-material runs (rarely longer than a row or
+               two) in rows of tz and the rest */
+            const int h = r.mixed ? std::min(tz, r.z + r.len - z) : r.len / k + (i < r.len % k ? 1 : 0);
+            emit(r.x0, z, h, r.mixed);
+            z += h;
+        }
+    }
+}
+
+/* walks the tiling of one mesh (owner slot o) at tile height tz */
+template <class F>
+void walk_tiles(const Slot &o, int tz, F emit)
+{
+    std::vector<ZRun> runs;
+    tile_runs(o, runs);
+    cut_runs(runs, tz, emit);
 }
 
 void build_tiles(const Slot &o, int slot, int tz, std::vector<int4> &t)
@@ -1692,32 +1720,96 @@ static int persistent_blocks(const semsks_engine *e, int n)
     return e->n_cu * per_cu;
 }
 
-/* Tile height before (or without) the autotuner: the tallest of 32, 16,
-   ..., 1 element rows that still gives every degree's launch at least two
-   tiles per wave of its persistent grid.  Many short tiles keep the grid of
-   a small mesh busy and its sweeps short (one 200x100-element instance:
-   0.020 ms per step at 1 row, 0.036 at 4); tall tiles read fewer halo rows
-   (64 instances on that mesh: 0.32 ms at 8-16 rows, 0.49 at 1). */
-static int default_tile_rows(const semsks_engine *e)
+/* Predicted duration of one step at tile height tz, in element-row sweeps
+   of one wave.  Each degree's launch deals its tiles to the earliest-free
+   of its W waves (the persistent grid's queues); a tile costs its rows,
+   0.7 for its halo row (no update), 0.5 for its start (the first fetch),
+   and a per-node-material tile 1.5 times that.  Tiles are dealt longest
+   first, whole rounds of W at a time, the rest one by one.  The model
+   ranks heights for the autotuner and picks the height of calls too short
+   to tune: short tiles keep the grid of a small mesh busy and its sweeps
+   short, tall tiles read fewer halo rows, and a height whose tile count
+   just fills a whole number of rounds of the grid leaves no wave idle at
+   the end of the step. */
+struct TileModel {
+    struct Mesh { int hpn; long copies; std::vector<ZRun> runs; };
+    std::vector<Mesh> meshes;       /* one per owner, with the slots that share it */
+    long waves[SEMSKS_MAXNP] = {0}; /* persistent grid of each degree's launch */
+    int maxz = 1;
+};
+
+static TileModel tile_model(const semsks_engine *e)
 {
-    int best = 32;
-    for (int n = 3; n <= 6; ++n) {
-        const long waves = 4L * persistent_blocks<false>(e, n);
-        const int xc = xchunk_of(n);
-        bool any = false;
-        int tz = 1;
-        for (int c = 32; c >= 1; c /= 2) {
-            long tiles = 0;
-            for (const Slot &sl : e->slots)
-                if (sl.used && sl.meta.hpn == n) {
-                    any = true;
-                    tiles += (long)((sl.meta.nex + xc - 1) / xc) * ((sl.meta.nez + c - 1) / c);
-                }
-            if (tiles >= 2 * waves) { tz = c; break; }
+    TileModel m;
+    std::map<int, size_t> at;
+    for (size_t k = 0; k < e->slots.size(); ++k) {
+        const Slot &s = e->slots[k];
+        if (!s.used) continue;
+        const int ow = s.share >= 0 ? s.share : (int)k;
+        auto it = at.find(ow);
+        if (it == at.end()) {
+            TileModel::Mesh me{s.meta.hpn, 0, {}};
+            tile_runs(e->slots[ow], me.runs);
+            it = at.emplace(ow, m.meshes.size()).first;
+            m.meshes.push_back(std::move(me));
         }
-        if (any) best = std::min(best, tz);
+        ++m.meshes[it->second].copies;
+        m.maxz = std::max(m.maxz, s.meta.nez);
+        if (!m.waves[s.meta.hpn]) m.waves[s.meta.hpn] = 4L * persistent_blocks<false>(e, s.meta.hpn);
+    }
+    return m;
+}
+
+static double predict_step(const TileModel &m, int tz)
+{
+    double total = 0.0;
+    std::vector<double> fin;
+    for (int n = 3; n <= 6; ++n) {
+        std::map<double, long> cnt;     /* tile cost -> tiles */
+        for (const auto &me : m.meshes) {
+            if (me.hpn != n) continue;
+            cut_runs(me.runs, tz, [&](int, int z0, int rows, int mixed) {
+                cnt[(rows + (z0 > 0 ? 0.7 : 0.0) + 0.5) * (mixed ? 1.5 : 1.0)] += me.copies;
+            });
+        }
+        if (cnt.empty()) continue;
+        const long W = m.waves[n];
+        fin.assign((size_t)W, 0.0);
+        for (auto kv = cnt.rbegin(); kv != cnt.rend(); ++kv) {
+            const double c = kv->first;
+            const long rounds = kv->second / W, rest = kv->second % W;
+            if (rounds)
+                for (double &f : fin) f += (double)rounds * c;
+            /* fin is kept ascending: the rest go to the earliest-free waves */
+            for (long i = 0; i < rest; ++i) fin[(size_t)i] += c;
+            std::sort(fin.begin(), fin.end());
+        }
+        total += fin.back();
+    }
+    return total;
+}
+
+/* the model's best height in [lo, hi] (the taller one on a tie) */
+static int model_tile_rows(const TileModel &m, int lo, int hi)
+{
+    hi = std::min(hi, m.maxz);
+    lo = std::min(lo, hi);
+    int best = hi;
+    double best_t = 1e300;
+    for (int tz = hi; tz >= lo; --tz) {
+        const double t = predict_step(m, tz);
+        if (t < best_t * (1.0 - 1e-9)) { best_t = t; best = tz; }
     }
     return best;
+}
+
+/* Tile height before (or without) the autotuner: the model's best from 1
+   to 32 rows (measured, one 200x100-element instance: 0.020 ms per step
+   at 1 row, 0.036 at 4; 64 instances on that mesh: 0.32 ms at 8-16 rows,
+   0.49 at 1). */
+static int default_tile_rows(const semsks_engine *e)
+{
+    return model_tile_rows(tile_model(e), 1, 32);
 }
 
 /* one launch of the element-force kernel for the tiles of polynomial degree n */
@@ -2311,16 +2403,17 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
     e->timed_steps = 0;
     int done = 0;
     /* autotuner: with tile_z unset, the first call that has the steps to
-       spare runs four real launches at each of the six candidate tile
-       heights and keeps the height whose fastest of the last three was
-       fastest (the tiling does not
-       change the arithmetic, so these steps are ordinary steps of the run).
-       Until then, and in calls too short for it (the serving loop's
-       rounds), default_tile_rows picks the height.  Measured: one
-       instance on the 200x100 mesh is fastest at 1 row, on the 2000x500
-       mesh at 4 (many tiles per wave balance the persistent grid); a
-       64-instance batch is nearly insensitive to the height */
-    static const int cand[] = {1, 2, 4, 8, 16, 32};
+       spare runs four real launches at each of six candidate tile heights
+       and keeps the height whose fastest of the last three was fastest
+       (the tiling does not change the arithmetic, so these steps are
+       ordinary steps of the run).  The candidates are the step model's
+       best height (predict_step) in each of the bands 1, 2-3, 4-7, 8-15,
+       16-31 and 32-64 rows.  Until then, and in calls too short for it
+       (the serving loop's rounds), default_tile_rows picks the height.
+       Measured: one instance on the 200x100 mesh is fastest at 1 row; a
+       64-instance batch on the 2000x500 mesh is nearly insensitive to the
+       height */
+    static const int band[][2] = {{1, 1}, {2, 3}, {4, 7}, {8, 15}, {16, 31}, {32, 64}};
     const int ncand = 6;
     if (e->opts.tile_z <= 0 && !e->tuned && nsteps >= 4 * ncand && sync != 2) {
         struct TuneFlag {
@@ -2328,6 +2421,11 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
             ~TuneFlag() { e->tuning = false; }
         } tune_flag{e};
         e->tuning = true;
+        int cand[ncand];
+        {
+            const TileModel m = tile_model(e);
+            for (int c = 0; c < ncand; ++c) cand[c] = model_tile_rows(m, band[c][0], band[c][1]);
+        }
         while ((int)e->tev.size() < 8) {
             hipEvent_t v;
             HIPCHK(hipEventCreate(&v));
@@ -2336,6 +2434,7 @@ semsks_status semsks_advance(semsks_engine *e, int nsteps, int sync)
         int best = e->tz_cur;
         float best_ms = 1e30f;
         for (int c = 0; c < ncand; ++c) {
+            if (std::find(cand, cand + c, cand[c]) != cand + c) continue;   /* bands that collapse on a low mesh */
             e->tz_cur = cand[c];
             semsks_status st = upload_tables(e);
             if (st) return st;
