@@ -101,7 +101,7 @@ def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0):
 
 
 def run_serving(rate, duration, nex_range, nez_range, nst_range, budget_gb=200.0, max_slots=1024, seed=None,
-                check_sample=True):
+                check_sample=True, tile_z=0):
     """Continuous batching of Poisson-arriving C4-style requests
     (semsks_serve): HBM-footprint admission against budget_gb, all admitted
     instances stepped together, each retired at its own last level.  Reports
@@ -125,7 +125,7 @@ def run_serving(rate, duration, nex_range, nez_range, nst_range, budget_gb=200.0
         pws.append((a[7], a[8], a[9], a[10]))
     log("  built %d requests in %.1fs" % (len(reqs), time.time() - t0))
     budget = int(budget_gb * 1e9)
-    with semsks.Engine(max_slots=max_slots, hbm_budget=budget) as e:
+    with semsks.Engine(max_slots=max_slots, hbm_budget=budget, tile_z=tile_z) as e:
         res, st = e.serve(probs, [a for a, _ in reqs], pws, steps_per_round=16, want_traces=True)
     ok = [r for r in res if r["status"] == 0]
     lat = np.array([r["latency"] for r in ok])
@@ -357,6 +357,9 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one device copy of this many MiB (PMC byte-counter calibration)")
+    ap.add_argument("--placement-spread", type=int, default=0,
+                    help="only time the batched step kernel in this many fresh engines one after another "
+                         "(same tiles; the spread comes from where each engine's allocations land)")
     ap.add_argument("--stream-ceiling", type=int, default=0,
                     help="only time a device copy and a triad a = b + 2c over arrays of this many MiB "
                          "(the practical HBM rate for 1:1 and 2:1 read:write traffic), and the step "
@@ -406,6 +409,22 @@ def main():
         print("bench.py runs on one GPU only (there is no multi-GPU path); got --gpus %d" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.placement_spread:
+        from semsks import configs
+        b3 = configs.Built(configs.c3_case(nst=10))
+        waves = [(b3, p, baz) for p, baz in configs.c3_waves()][: args.batch]
+        tz, res = args.tile_z, []
+        for i in range(args.placement_spread):
+            # the first engine autotunes when no height is given; the rest reuse its pick
+            r = run_engine(waves, 4, 4 if tz else 24, True, tile_z=tz, extra_steps=3 * args.kernel_steps)
+            tz = tz or r["tile_rows"]
+            ks = [kernel_time(r["engine"], args.kernel_steps)[0] for _ in range(3)]
+            r["engine"].close()
+            res.append({"kernel_ms": ks, "tile_rows": tz})
+            log("  engine %d: step kernel %s ms" % (i, " ".join("%.3f" % k for k in ks)))
+        print(json.dumps({"workload": "batched step kernel, %d plane waves on the 2000x500 mesh, one fresh engine "
+                                      "after another" % len(waves), "engines": res}))
+        return
 
     import semsks
     from semsks import configs
@@ -450,6 +469,12 @@ def main():
         r1b["engine"].close()
         log("  C1 plain %.3e (tiles of %d rows), batched x%d %.3e pt-steps/s (%d rows)" % (
             r1p["throughput"], r1p["tile_rows"], len(w1), r1b["throughput"], r1b["tile_rows"]))
+        # a batch that fills the GPU on this mesh: the 64 waves four times over
+        # (ray parameters shifted by 1e-4 s/km per copy)
+        w256 = [(b1, p + 1e-4 * (k // len(w1)), baz) for k, (_, p, baz) in enumerate(w1 * 4)]
+        r1c = run_engine(w256, args.steps, args.warmup, True, tile_z=args.tile_z)
+        r1c["engine"].close()
+        log("  C1 batched x%d %.3e pt-steps/s (%d rows)" % (len(w256), r1c["throughput"], r1c["tile_rows"]))
         B = alg_bytes(9, len(waves))
         achieved = B * r3["npts"] / (k3 * 1e-3) / 1e9
         value, ms, npts, steps = r3["throughput"], r3["ms_per_step"], r3["npts"], args.steps
@@ -474,6 +499,8 @@ def main():
                "c1_mesh_tile_rows_plain": r1p["tile_rows"], "c1_mesh_tile_rows_batched": r1b["tile_rows"],
                "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
                "c1_mesh_workload": "64 plane S waves on the 200x100-element mesh vs one",
+               "c1_mesh_batched256_value": r1c["throughput"], "c1_mesh_tile_rows_batched256": r1c["tile_rows"],
+               "c1_mesh_batched256_over_plain": r1c["throughput"] / r1p["throughput"],
                "bytes_per_point_step_alg": B, "kahan": True}
         if args.c4:
             log("c4: uneven batch of 96 instances")
@@ -492,7 +519,7 @@ def main():
             cfg["c4"] = c4
         if args.serve_window > 0:
             log("serving: Poisson 20/s for %.0f s, C5 requests" % args.serve_window)
-            sv = run_serving(20.0, args.serve_window, (200, 1600), (100, 400), (2000, 12000))
+            sv = run_serving(20.0, args.serve_window, (200, 1600), (100, 400), (2000, 12000), tile_z=args.tile_z)
             log("  %d of %d requests done, p50 %.3f s, p99 %.3f s, %.3e pt-steps/s" % (
                 sv["completed"], sv["requests"], sv["p50_latency_s"], sv["p99_latency_s"], sv["throughput"]))
             cfg["serving"] = sv
@@ -535,7 +562,7 @@ def main():
         # the serving configuration over a bounded arrival window (--serve-window;
         # the stated 300 s of 20 requests/s offers several times what one GPU
         # serves, see load_over_served): C4-size requests, 200 GB budget
-        sv = run_serving(20.0, args.serve_window, (200, 1600), (100, 400), (2000, 12000))
+        sv = run_serving(20.0, args.serve_window, (200, 1600), (100, 400), (2000, 12000), tile_z=args.tile_z)
         value, ms, steps = sv["throughput"], float("nan"), 0
         achieved, traffic = float("nan"), None
         cfg = {"workload": "serving: Poisson 20 requests/s for %.0f s of arrivals, C4-style instances, 200 GB HBM "

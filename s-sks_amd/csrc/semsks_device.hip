@@ -1803,13 +1803,38 @@ static int model_tile_rows(const TileModel &m, int lo, int hi)
     return best;
 }
 
-/* Tile height before (or without) the autotuner: the model's best from 1
-   to 32 rows (measured, one 200x100-element instance: 0.020 ms per step
-   at 1 row, 0.036 at 4; 64 instances on that mesh: 0.32 ms at 8-16 rows,
-   0.49 at 1). */
+/* Tile height before (or without) the autotuner: the tallest of 32, 16,
+   ..., 1 element rows that still gives every degree's launch at least two
+   tiles per wave of its persistent grid.  Many short tiles keep the grid of
+   a small mesh busy and its sweeps short (one 200x100-element instance:
+   0.020 ms per step at 1 row, 0.036 at 4); tall tiles read fewer halo rows
+   (64 instances on that mesh: 0.32 ms at 8-16 rows, 0.49 at 1).  In the
+   serving loop the instance set changes with every admission and
+   retirement; this rule moves between few heights, where the step model's
+   best height would move at nearly every change and re-tile and re-capture
+   the step graphs each time (the C5 window: 8.5e10 point-steps/s against
+   7.2e10 with the model's height and 8.1e10 with the model restricted to
+   powers of two). */
 static int default_tile_rows(const semsks_engine *e)
 {
-    return model_tile_rows(tile_model(e), 1, 32);
+    int best = 32;
+    for (int n = 3; n <= 6; ++n) {
+        const long waves = 4L * persistent_blocks<false>(e, n);
+        const int xc = xchunk_of(n);
+        bool any = false;
+        int tz = 1;
+        for (int c = 32; c >= 1; c /= 2) {
+            long tiles = 0;
+            for (const Slot &sl : e->slots)
+                if (sl.used && sl.meta.hpn == n) {
+                    any = true;
+                    tiles += (long)((sl.meta.nex + xc - 1) / xc) * ((sl.meta.nez + c - 1) / c);
+                }
+            if (tiles >= 2 * waves) { tz = c; break; }
+        }
+        if (any) best = std::min(best, tz);
+    }
+    return best;
 }
 
 /* one launch of the element-force kernel for the tiles of polynomial degree n */
