@@ -259,6 +259,32 @@ typedef struct semsks_serve_stats {
 semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int steps_per_round,
                            semsks_serve_stats *stats);
 
+/* Admission order of semsks_serve_ex among the requests that have arrived
+   and wait: earliest arrival first, or least work (nglob * (nst - 1))
+   first, earliest arrival on a tie. */
+#define SEMSKS_SERVE_FCFS 0
+#define SEMSKS_SERVE_SHORTEST 1
+
+typedef struct semsks_serve_opts {
+    int steps_per_round;            /* steps of every round (>= 1) */
+    int order;                      /* SEMSKS_SERVE_FCFS or SEMSKS_SERVE_SHORTEST */
+    double max_active_points;       /* admit only while the active instances hold at most this many
+                                       GLL points (0: no cap; an idle engine always admits) */
+} semsks_serve_opts;
+
+/* semsks_serve with an admission policy (no counterpart in the reference,
+   which runs one solve per process).  Under a load above what the GPU
+   serves, admitting everything that fits the HBM budget makes every
+   admitted request share the device until the backlog drains; a cap on the
+   active points that still fills the GPU, with the least work first,
+   finishes short requests early at the same throughput.  Admission stops at
+   the first request that does not fit (head of line in the chosen order),
+   so with SEMSKS_SERVE_SHORTEST a large request waits while smaller ones
+   keep arriving: under an unbounded overload it can wait without bound.
+   semsks_serve is semsks_serve_ex with FCFS and no cap. */
+semsks_status semsks_serve_ex(semsks_engine *e, semsks_request *req, int n, const semsks_serve_opts *opts,
+                              semsks_serve_stats *stats);
+
 /* ---------------------------------------------------------------------- */
 /* Single-run convenience (what the CLI uses)                              */
 /* ---------------------------------------------------------------------- */

@@ -1804,7 +1804,7 @@ static int model_tile_rows(const TileModel &m, int lo, int hi)
 }
 
 /* Tile height before (or without) the autotuner: the tallest of 32, 16,
-   ..., 1 element rows that still gives every degree's launch at least two
+   ..., 1 element rows that still gives every degree's launch at least four
    tiles per wave of its persistent grid.  Many short tiles keep the grid of
    a small mesh busy and its sweeps short (one 200x100-element instance:
    0.020 ms per step at 1 row, 0.036 at 4); tall tiles read fewer halo rows
@@ -1814,7 +1814,9 @@ static int model_tile_rows(const TileModel &m, int lo, int hi)
    best height would move at nearly every change and re-tile and re-capture
    the step graphs each time (the C5 window: 8.5e10 point-steps/s against
    7.2e10 with the model's height and 8.1e10 with the model restricted to
-   powers of two). */
+   powers of two).  Four tiles per wave rather than two keep the grid's
+   tail short when the serving loop caps the active points (the C5 window
+   with at most 64M active points: 8.07e10 at 16 rows, 7.71e10 at 32). */
 static int default_tile_rows(const semsks_engine *e)
 {
     int best = 32;
@@ -1830,7 +1832,7 @@ static int default_tile_rows(const semsks_engine *e)
                     any = true;
                     tiles += (long)((sl.meta.nex + xc - 1) / xc) * ((sl.meta.nez + c - 1) / c);
                 }
-            if (tiles >= 2 * waves) { tz = c; break; }
+            if (tiles >= 4 * waves) { tz = c; break; }
         }
         if (any) best = std::min(best, tz);
     }

@@ -50,10 +50,30 @@ static semsks_status try_admit(semsks_engine *e, semsks_request *r, int *slot)
     return semsks_admit(e, p, -1, slot);
 }
 
+/* GLL points of one instance and the point-steps of its whole run */
+static double points_of(const semsks_problem *p)
+{
+    return (double)(p->nex * p->hpn + 1) * (double)(p->nez * p->hpn + 1);
+}
+static double work_of(const semsks_problem *p) { return points_of(p) * (double)(p->nst - 1); }
+
 semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int steps_per_round,
                            semsks_serve_stats *stats)
 {
-    if (!e || (n > 0 && !req) || steps_per_round < 1) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
+    semsks_serve_opts o;
+    memset(&o, 0, sizeof o);
+    o.steps_per_round = steps_per_round;
+    o.order = SEMSKS_SERVE_FCFS;
+    return semsks_serve_ex(e, req, n, &o, stats);
+}
+
+semsks_status semsks_serve_ex(semsks_engine *e, semsks_request *req, int n, const semsks_serve_opts *o,
+                              semsks_serve_stats *stats)
+{
+    if (!e || (n > 0 && !req) || !o || o->steps_per_round < 1 ||
+        (o->order != SEMSKS_SERVE_FCFS && o->order != SEMSKS_SERVE_SHORTEST) || !(o->max_active_points >= 0.0))
+        return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
+    const int steps_per_round = o->steps_per_round;
     for (int i = 1; i < n; ++i)
         if (req[i].arrival_s < req[i - 1].arrival_s)
             return semsks_set_error(SEMSKS_EINVAL, "requests must be sorted by arrival time");
@@ -65,19 +85,20 @@ semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int ste
         req[i].slot = -1;
         req[i].status = SEMSKS_OK;
     }
-    int *queue = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    /* wait: requests that have arrived and are not admitted, in arrival order */
+    int *wait = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
     int *active = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
-    if (!queue || !active) {
-        free(queue); free(active);
+    if (!wait || !active) {
+        free(wait); free(active);
         return semsks_set_error(SEMSKS_EINVAL, "out of host memory");
     }
-    int qh = 0, qt = 0, nact = 0, next = 0, finished = 0, max_active = 0, rounds = 0;
-    double point_steps = 0.0;
+    int nwait = 0, nact = 0, next = 0, finished = 0, max_active = 0, rounds = 0;
+    double point_steps = 0.0, act_points = 0.0;
     semsks_status st = SEMSKS_OK;
     const double t0 = now_s();
     while (finished < n) {
         const double t = now_s() - t0;
-        while (next < n && req[next].arrival_s <= t) queue[qt++] = next++;
+        while (next < n && req[next].arrival_s <= t) wait[nwait++] = next++;
         /* the next round of the admitted instances goes out first ... */
         const int launched = nact > 0;
         if (launched) {
@@ -85,30 +106,36 @@ semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int ste
             if (st != SEMSKS_OK) break;
             ++rounds;
         }
-        /* ... and the admissions, first come first served, overlap it: their
-           host work, and their device work on the engine's admission stream */
-        while (qh < qt) {
-            semsks_request *r = &req[queue[qh]];
+        /* ... and the admissions overlap it: their host work, and their
+           device work on the engine's admission stream.  The next request
+           is the earliest arrival (FCFS) or the one with the least work
+           (SHORTEST, earliest arrival on a tie); admission stops at the
+           first that does not fit, or that would take the active instances
+           past max_active_points (never when none is active) */
+        while (nwait > 0) {
+            int w = 0;
+            if (o->order == SEMSKS_SERVE_SHORTEST)
+                for (int j = 1; j < nwait; ++j)
+                    if (work_of(req[wait[j]].problem) < work_of(req[wait[w]].problem)) w = j;
+            semsks_request *r = &req[wait[w]];
+            const double pts = points_of(r->problem);
+            if (o->max_active_points > 0.0 && nact > 0 && act_points + pts > o->max_active_points) break;
             int slot = -1;
             semsks_status a = try_admit(e, r, &slot);
-            if (a == SEMSKS_ENOHBM || a == SEMSKS_ENOSLOT) {
-                if (nact == 0) {              /* cannot fit even into an idle engine */
-                    r->status = a;
-                    r->done_s = now_s() - t0;
-                    ++qh; ++finished;
-                    continue;
-                }
-                break;
-            }
-            if (a != SEMSKS_OK) {
+            if ((a == SEMSKS_ENOHBM || a == SEMSKS_ENOSLOT) && nact > 0) break;
+            const int k = wait[w];
+            memmove(wait + w, wait + w + 1, sizeof(int) * (size_t)(nwait - w - 1));
+            --nwait;
+            if (a != SEMSKS_OK) {          /* cannot fit even into an idle engine, or failed */
                 r->status = a;
                 r->done_s = now_s() - t0;
-                ++qh; ++finished;
+                ++finished;
                 continue;
             }
             r->slot = slot;
             r->admit_s = now_s() - t0;
-            active[nact++] = queue[qh++];
+            active[nact++] = k;
+            act_points += pts;
             if (nact > max_active) max_active = nact;
         }
         if (!launched) {
@@ -131,9 +158,10 @@ semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int ste
                 if (st != SEMSKS_OK) break;
             }
             r->done_s = now_s() - t0;
-            point_steps += (double)(p->nex * p->hpn + 1) * (double)(p->nez * p->hpn + 1) * (double)(p->nst - 1);
+            point_steps += work_of(p);
             st = semsks_retire(e, r->slot);
             if (st != SEMSKS_OK) break;
+            act_points -= points_of(p);
             active[i] = active[--nact];
             ++finished;
         }
@@ -149,7 +177,7 @@ semsks_status semsks_serve(semsks_engine *e, semsks_request *req, int n, int ste
     }
     /* on a device error, whatever is still admitted is retired */
     for (int i = 0; i < nact; ++i) semsks_retire(e, req[active[i]].slot);
-    free(queue);
+    free(wait);
     free(active);
     return st;
 }

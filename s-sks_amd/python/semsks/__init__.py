@@ -30,7 +30,8 @@ EXPORTS = [
     "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_admission_bytes", "semsks_advance", "semsks_prepare",
     "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_retire", "semsks_sync",
     "semsks_last_timing", "semsks_last_timed_steps", "semsks_sweep_steps", "semsks_tile_rows", "semsks_run_problem", "semsks_stiff", "semsks_stiff_dev",
-    "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve", "semsks_uvw2sac",
+    "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve", "semsks_serve_ex",
+    "semsks_uvw2sac",
 ]
 
 
@@ -62,6 +63,13 @@ class Request(C.Structure):
 class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("point_steps", C.c_double), ("completed", C.c_int),
                 ("max_active", C.c_int), ("rounds", C.c_int)]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("steps_per_round", C.c_int), ("order", C.c_int), ("max_active_points", C.c_double)]
+
+
+SERVE_ORDER = {"fcfs": 0, "shortest": 1}
 
 
 class EngineOpts(C.Structure):
@@ -117,6 +125,7 @@ def lib() -> C.CDLL:
     L.semsks_planewave.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float,
                                    C.c_double, C.c_double, C.c_double, C.c_double, vp, vp]
     L.semsks_serve.argtypes = [vp, C.POINTER(Request), C.c_int, C.c_int, C.POINTER(ServeStats)]
+    L.semsks_serve_ex.argtypes = [vp, C.POINTER(Request), C.c_int, C.POINTER(ServeOpts), C.POINTER(ServeStats)]
     _lib = L
     return L
 
@@ -274,15 +283,19 @@ class Engine:
         _check(lib().semsks_planewave(self._h, nex, nez, hpn, neh, eh, nt, dp, p, beta, f0, psi, None, None))
         return None
 
-    def serve(self, problems, arrivals, planewaves=None, steps_per_round: int = 16, want_traces: bool = True):
-        """Continuous batching (semsks_serve): problems[i] arrives arrivals[i]
-        seconds after the start; planewaves[i] = (p, beta, f0, psi) generates
-        its boundary records on the device.  Returns (per-request dicts, stats)."""
+    def serve(self, problems, arrivals, planewaves=None, steps_per_round: int = 16, want_traces: bool = True,
+              order: str = "fcfs", max_active_points: float = 0.0):
+        """Continuous batching (semsks_serve_ex): problems[i] arrives
+        arrivals[i] seconds after the start; planewaves[i] = (p, beta, f0,
+        psi) generates its boundary records on the device; order "fcfs" or
+        "shortest" (least work first) among the waiting requests, admitted
+        while the active instances hold at most max_active_points GLL points
+        (0: no cap).  Returns (per-request dicts, stats)."""
         n = len(problems)
-        order = sorted(range(n), key=lambda i: arrivals[i])
+        by_arrival = sorted(range(n), key=lambda i: arrivals[i])
         reqs = (Request * n)()
         keep = []
-        for j, i in enumerate(order):
+        for j, i in enumerate(by_arrival):
             p = problems[i]
             r = reqs[j]
             r.problem = C.pointer(p)
@@ -298,9 +311,11 @@ class Engine:
             else:
                 keep.append(None)
         st = ServeStats()
-        _check(lib().semsks_serve(self._h, reqs, n, steps_per_round, C.byref(st)))
+        opts = ServeOpts(steps_per_round=steps_per_round, order=SERVE_ORDER[order],
+                         max_active_points=float(max_active_points))
+        _check(lib().semsks_serve_ex(self._h, reqs, n, C.byref(opts), C.byref(st)))
         out = [None] * n
-        for j, i in enumerate(order):
+        for j, i in enumerate(by_arrival):
             r = reqs[j]
             out[i] = dict(arrival=r.arrival_s, admit=r.admit_s, done=r.done_s, status=r.status, slot=r.slot,
                           latency=r.done_s - r.arrival_s, traces=keep[j])

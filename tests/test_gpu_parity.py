@@ -231,6 +231,42 @@ def test_serving_results_equal_alone_and_in_the_mix(tmp_path, monkeypatch):
             assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, c, l2, pk)
 
 
+def test_serving_shortest_first_under_a_point_cap(tmp_path, monkeypatch):
+    """semsks_serve_ex with least-work-first admission and a cap on the
+    active GLL points: requests that wait are admitted smallest work first,
+    the active instances never hold more than the cap (an idle engine admits
+    whatever is next), and every result equals its run alone, bit for bit."""
+    print("case: serving, least work first, point cap", flush=True)
+    specs = [dict(nex=36, nst=300, p=0.06), dict(nex=24, nst=120, p=0.05, psi_deg=40.0, sh_zero=False),
+             dict(nex=30, nst=200, p=0.07, solver=1), dict(nex=20, nst=90, p=0.045),
+             dict(nex=28, nst=260, p=0.055, psi_deg=70.0, sh_zero=False)]
+    models = []
+    for k, kw in enumerate(specs):
+        d = make_case(tmp_path / ("r%d" % k), **kw)
+        monkeypatch.chdir(d)
+        models.append(semsks.Model(["par=run.par"]))
+    probs = [m.problem for m in models]
+    solo = [run_gpu(p) for p in probs]
+    pts = [(p.nex * p.hpn + 1) * (p.nez * p.hpn + 1) for p in probs]
+    work = [n * (p.nst - 1) for n, p in zip(pts, probs)]
+    cap = max(pts) + min(pts)          # room for the largest and one more at most
+    with semsks.Engine(max_slots=8) as e:
+        res, stats = e.serve(probs, [0.0] * len(probs), steps_per_round=16, order="shortest", max_active_points=cap)
+    assert stats["completed"] == len(probs)
+    for r, s in zip(res, solo):
+        assert r["status"] == 0 and r["done"] >= r["admit"] >= r["arrival"]
+        assert np.array_equal(r["traces"].view(np.uint32), s.view(np.uint32))
+    # admitted in order of work (all arrived at once)
+    by_admit = sorted(range(len(probs)), key=lambda i: res[i]["admit"])
+    assert by_admit == sorted(range(len(probs)), key=lambda i: (work[i], i))
+    # at every admission the active points stay within the cap, unless the
+    # request was admitted into an idle engine
+    for i in range(len(probs)):
+        t = res[i]["admit"]
+        active = [j for j in range(len(probs)) if res[j]["admit"] <= t < res[j]["done"]]
+        assert sum(pts[j] for j in active) <= cap or active == [i], (i, active)
+
+
 def test_serving_waits_for_memory_and_rejects_what_never_fits():
     """Admission by HBM footprint: with a budget for two of the small
     requests, the third waits until one retires; a request larger than the

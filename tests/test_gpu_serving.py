@@ -46,7 +46,7 @@ def test_c5_window_served_and_sample_matches_fp64():
     assert st["completed"] == len(reqs)
     for r in res:
         assert r["status"] == 0 and np.isfinite(r["traces"]).all()
-    worst = {}
+    worst, quiet = {}, []
     held = [i for i in sample if "r%d_nst" % i in ref.files]     # a fixture still being written holds a prefix
     print("sampled requests with FP64 traces: %s of %s" % (held, sample), flush=True)
     assert held
@@ -56,6 +56,14 @@ def test_c5_window_served_and_sample_matches_fp64():
         for k, key in ((0, "u"), (1, "mw"), (2, "v")):
             a, b = tr[:, k].astype(np.float64), ref["r%d_%s" % (i, key)].astype(np.float64)
             assert a.shape == b.shape
+            if np.abs(b).max() < 1e-6:
+                # nothing of this component reaches the surface within the
+                # request's steps (a deep mesh: the S-to-P conversions at an
+                # interface arrive, the direct S and with it all of SH after
+                # the last step): the FP64 record is zero, the FP32 one must be
+                assert np.abs(a).max() < 1e-5, (i, key, float(np.abs(a).max()))
+                quiet.append((i, key))
+                continue
             # receivers the incident wave has not reached by the last step
             # hold (near) zero: a floor of 1e-3 of the largest receiver's
             # norm and peak keeps them from dividing by ~0
@@ -65,7 +73,8 @@ def test_c5_window_served_and_sample_matches_fp64():
             worst[(i, key)] = (float(l2.max()), float(pk.max()))
             assert np.abs(b).max() > 0.05
             assert l2.max() <= 1e-4 and pk.max() <= 1e-4, (i, key, l2.max(), pk.max())
-    print("worst rel-L2, peak misfit per sampled request:", worst, flush=True)
+    assert worst
+    print("worst rel-L2, peak misfit per sampled request:", worst, "; no arrival at the receivers:", quiet, flush=True)
     small = sorted(sample, key=lambda i: built[i].nglob() * built[i].case.nst)[:2]
     for i in small:
         with semsks.Engine() as e:
