@@ -168,7 +168,7 @@ def serving_sample_check(reqs, res):
     dec = int(ref["decim"])
     rows, worst, quiet = [], 0.0, []
     for i in sample:
-        if "r%d_u" % i not in ref.files or res[i]["status"] != 0:
+        if "r%d_u" % i not in ref.files or res[i]["status"] != 0 or int(ref["r%d_nst" % i]) != reqs[i][1].nst:
             continue
         tr = res[i]["traces"][::dec]
         row = {"request": i}
@@ -362,6 +362,8 @@ def main():
     ap.add_argument("--serve-window", type=float, default=3.0,
                     help="seconds of Poisson arrivals of C5-size requests in the serving leg of the default "
                          "run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0,
+                    help="Poisson arrival rate of the serving leg, requests/s (the stated C5 load: 20)")
     ap.add_argument("--serve-order", choices=["fcfs", "shortest"], default="shortest",
                     help="admission order of the serving leg among waiting requests (least work first by "
                          "default; the default run also reports first come first served with no cap)")
@@ -535,8 +537,8 @@ def main():
                                                                       c4["kernel_ms"]))
             cfg["c4"] = c4
         if args.serve_window > 0:
-            log("serving: Poisson 20/s for %.0f s, C5 requests" % args.serve_window)
-            sv = run_serving(20.0, args.serve_window, (200, 1600), (100, 400), (2000, 12000), tile_z=args.tile_z,
+            log("serving: Poisson %g/s for %.0f s, C5 requests" % (args.serve_rate, args.serve_window))
+            sv = run_serving(args.serve_rate, args.serve_window, (200, 1600), (100, 400), (2000, 12000), tile_z=args.tile_z,
                              order=args.serve_order, max_active_points=args.serve_cap)
             log("  %d of %d requests done, p50 %.3f s, p99 %.3f s, %.3e pt-steps/s" % (
                 sv["completed"], sv["requests"], sv["p50_latency_s"], sv["p99_latency_s"], sv["throughput"]))
@@ -544,7 +546,7 @@ def main():
             if (args.serve_order, args.serve_cap) != ("fcfs", 0.0):
                 # the same stream through the plain loop: every arrival admitted
                 # as soon as it fits, all admitted instances stepped together
-                sf = run_serving(20.0, args.serve_window, (200, 1600), (100, 400), (2000, 12000),
+                sf = run_serving(args.serve_rate, args.serve_window, (200, 1600), (100, 400), (2000, 12000),
                                  tile_z=args.tile_z, check_sample=False)
                 log("  first come first served, no cap: p50 %.3f s, p99 %.3f s, %.3e pt-steps/s" % (
                     sf["p50_latency_s"], sf["p99_latency_s"], sf["throughput"]))
@@ -590,12 +592,12 @@ def main():
         # the serving configuration over a bounded arrival window (--serve-window;
         # the stated 300 s of 20 requests/s offers several times what one GPU
         # serves, see load_over_served): C4-size requests, 200 GB budget
-        sv = run_serving(20.0, args.serve_window, (200, 1600), (100, 400), (2000, 12000), tile_z=args.tile_z,
+        sv = run_serving(args.serve_rate, args.serve_window, (200, 1600), (100, 400), (2000, 12000), tile_z=args.tile_z,
                              order=args.serve_order, max_active_points=args.serve_cap)
         value, ms, steps = sv["throughput"], float("nan"), 0
         achieved, traffic = float("nan"), None
-        cfg = {"workload": "serving: Poisson 20 requests/s for %.0f s of arrivals, C4-style instances, 200 GB HBM "
-                           "budget" % args.serve_window, "serving": sv}
+        cfg = {"workload": "serving: Poisson %g requests/s for %.0f s of arrivals, C4-style instances, 200 GB HBM "
+                           "budget" % (args.serve_rate, args.serve_window), "serving": sv}
         cpu_case = lambda: configs.c2_case(nst=10)
     elif args.config == "c1b":
         # the literal reading: 64 waves on the C1 mesh (321,201 points each)
