@@ -15,7 +15,7 @@ The serving leg of bench.py and tests/test_gpu_serving.py compare the FP32
 traces each sampled request gets from the continuous-batching loop against
 these.
 
-  python tests/golden/make_c5_sample_fp64.py     (about an hour on 8 cores)
+  python tests/golden/make_c5_sample_fp64.py     (about two hours on 8 cores)
 """
 import os
 import sys

@@ -46,7 +46,7 @@ def test_c5_window_served_and_sample_matches_fp64():
     assert st["completed"] == len(reqs)
     for r in res:
         assert r["status"] == 0 and np.isfinite(r["traces"]).all()
-    worst, quiet = {}, []
+    worst, quiet, peaks = {}, [], []
     held = [i for i in sample if "r%d_nst" % i in ref.files]     # a fixture still being written holds a prefix
     print("sampled requests with FP64 traces: %s of %s" % (held, sample), flush=True)
     assert held
@@ -71,9 +71,11 @@ def test_c5_window_served_and_sample_matches_fp64():
             l2 = np.linalg.norm(a - b, axis=0) / np.maximum(nb, 1e-3 * nb.max())
             pk = np.abs(a - b).max(axis=0) / np.maximum(pb, 1e-3 * pb.max())
             worst[(i, key)] = (float(l2.max()), float(pk.max()))
-            assert np.abs(b).max() > 0.05
+            peaks.append(float(np.abs(b).max()))
             assert l2.max() <= 1e-4 and pk.max() <= 1e-4, (i, key, l2.max(), pk.max())
-    assert worst
+    # the check is not vacuous: some checked component carries the incident
+    # pulse (peak 1) or more, however weak the others are (converted phases only)
+    assert max(peaks) > 0.5
     print("worst rel-L2, peak misfit per sampled request:", worst, "; no arrival at the receivers:", quiet, flush=True)
     small = sorted(sample, key=lambda i: built[i].nglob() * built[i].case.nst)[:2]
     for i in small:
