@@ -494,6 +494,12 @@ def main():
         r1c = run_engine(w256, args.steps, args.warmup, True, tile_z=args.tile_z)
         r1c["engine"].close()
         log("  C1 batched x%d %.3e pt-steps/s (%d rows)" % (len(w256), r1c["throughput"], r1c["tile_rows"]))
+        # and sixteen times over: the batch size at which this mesh's batch
+        # rate levels off (64 waves fill about a fifth of the GPU's waves)
+        w1024 = [(b1, p + 1e-4 * (k // len(w1)), baz) for k, (_, p, baz) in enumerate(w1 * 16)]
+        r1d = run_engine(w1024, args.steps, args.warmup, True, tile_z=args.tile_z)
+        r1d["engine"].close()
+        log("  C1 batched x%d %.3e pt-steps/s (%d rows)" % (len(w1024), r1d["throughput"], r1d["tile_rows"]))
         B = alg_bytes(9, len(waves))
         achieved = B * r3["npts"] / (k3 * 1e-3) / 1e9
         value, ms, npts, steps = r3["throughput"], r3["ms_per_step"], r3["npts"], args.steps
@@ -506,7 +512,9 @@ def main():
                "batched_over_plain": r3["throughput"] / rp["throughput"],
                "batched_over_plain_note": "on the 16M-point mesh one instance already fills the GPU: 5x the plain "
                                           "rate would need %.1f TB/s at the batch's 36.6 B/pt, over the 8 TB/s HBM "
-                                          "peak; the ratio on the 200x100 mesh is c1_mesh_batched_over_plain"
+                                          "peak; on the 200x100 mesh see c1_mesh_batched_over_plain (64 waves), "
+                                          "c1_mesh_batched256_over_plain and c1_mesh_batched1024_over_plain (the batch "
+                                          "size at which that mesh's rate levels off)"
                                           % (5 * rp["throughput"] * B / 1e12),
                "plain_hbm_bound": hbm_bound_plain,
                "kernel_ms_batched": k3, "kernel_ms_plain": kp,
@@ -520,6 +528,8 @@ def main():
                "c1_mesh_workload": "64 plane S waves on the 200x100-element mesh vs one",
                "c1_mesh_batched256_value": r1c["throughput"], "c1_mesh_tile_rows_batched256": r1c["tile_rows"],
                "c1_mesh_batched256_over_plain": r1c["throughput"] / r1p["throughput"],
+               "c1_mesh_batched1024_value": r1d["throughput"], "c1_mesh_tile_rows_batched1024": r1d["tile_rows"],
+               "c1_mesh_batched1024_over_plain": r1d["throughput"] / r1p["throughput"],
                "bytes_per_point_step_alg": B, "kahan": True}
         if args.c4:
             log("c4: uneven batch of 96 instances")
