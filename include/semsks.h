@@ -117,6 +117,15 @@ void semsks_model_dims(const semsks_model *m, int dims[8]);
 const char *semsks_model_outdir(const semsks_model *m);
 void semsks_model_free(semsks_model *m);
 
+/* The setup diagnostics the reference writes with check=1, its default
+   (sem2d_psvsh.c:47,179): grid.xz, rho on every second node column and row
+   (:247-260); legendre.xz, the Lagrange basis on [-1, 1] in steps of 0.05
+   (:262-270, lagrange.c:20-32,323-366); vel.xy, per node x in degrees of
+   epicentral distance from xmin, depth and vp = sqrt(c11/rho), or the
+   splitting (v44 - v55)/(v44 + v55)/2 with output_vd=1 (discret, :804-825).
+   Written into the model's outdir before the solve; nothing when check=0. */
+semsks_status semsks_write_check(const semsks_model *m);
+
 /* Writes seis.u / seis.w / seis.v (nst records of ntrace float32 each) from
    a trace block [nst][3][ntrace] ordered u, -w, v (kirrecord_zhl,
    sem2d_psvsh.c:2116-2165). */

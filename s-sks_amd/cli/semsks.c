@@ -35,6 +35,14 @@ int main(int argc, char **argv)
     semsks_model_dims(m, dims);
     printf("nx=%d nz=%d nex=%d nez=%d eh=%f ntrace=%d nst=%d nt_sem=%d materials=%d\n", dims[0], dims[1], p->nex,
            p->nez, p->eh, p->ntrace, p->nst, dims[4], dims[7]);
+    /* check=1 (the default): the setup diagnostics, before the solve as in
+       the reference (sem2d_psvsh.c:247-270, discret :804-825) */
+    st = semsks_write_check(m);
+    if (st != SEMSKS_OK) {
+        fprintf(stderr, "semsks: %s\n", semsks_last_error());
+        semsks_model_free(m);
+        return exit_code(st);
+    }
     size_t n = (size_t)p->nst * 3 * (p->ntrace > 0 ? p->ntrace : 1);
     float *tr = (float *)calloc(n, sizeof(float));
     st = semsks_run_problem(p, tr);

@@ -222,6 +222,8 @@ static void to_record(const consts *a, float *r)
 struct semsks_model {
     semsks_problem prob;
     int nx, nz, nt_sem, plot_trace;
+    int check, output_vd;           /* check=1 (the default): the setup diagnostics files */
+    float xmin;                     /* epicentral distance of the mesh's x = 0, km */
     char outdir[1024];
     float *mat_table;
     uint16_t *mat_id;
@@ -539,6 +541,9 @@ semsks_status semsks_load_par(int argc, char **argv, semsks_model **out)
     get_f(&P, "zbeg", &zbeg);
     get_i(&P, "get_rec", &get_rec);
     get_i(&P, "plot_trace", &M->plot_trace);
+    M->check = 1;                   /* sem2d_psvsh.c:47,179 */
+    get_i(&P, "check", &M->check);
+    get_i(&P, "output_vd", &M->output_vd);
     if (get_rec == 1) req_s(&P, "rec_file", rec_file, sizeof rec_file);
     req_i(&P, "idxplot", &idxplot);
     req_i(&P, "msource", &msource);
@@ -565,6 +570,7 @@ semsks_status semsks_load_par(int argc, char **argv, semsks_model **out)
     float *cc = NULL, *ss = NULL, *dd = NULL, *tth = NULL, thk = 0;
     st = read_raymodel(raymodel, flat, nfil, eh, &nez, &eh, &nrec, &jo, &cc, &ss, &dd, &tth, &thk);
     if (st != SEMSKS_OK) { semsks_model_free(M); return st; }
+    M->xmin = xmin;
     p->nex = nex; p->nez = nez; p->hpn = hpn; p->neh = neh; p->eh = eh; p->sdt = sdt; p->nst = nst;
     p->solver = solver; p->absorb = absorb; p->flags = kahan ? 0 : SEMSKS_F_NO_KAHAN;
     p->atten = atten; p->ass = ass;
@@ -648,3 +654,101 @@ semsks_status semsks_write_seis(const char *outdir, int nst, int ntrace, const f
     }
     return SEMSKS_OK;
 }
+
+/* ---------------------------------------------------------------------- */
+/* check=1 diagnostics: grid.xz, legendre.xz, vel.xy                       */
+/* ---------------------------------------------------------------------- */
+/* GLL nodes in float as the reference computes them (lagrange.c:323-366:
+   closed forms in double, stored to float; for degrees 5 and 6 the
+   quadratic's coefficients are floats, so c*c-4*a*e is a float product) */
+static void gll_float(int n, float *x)
+{
+    x[0] = -1.f;
+    x[n] = 1.f;
+    if (n == 3) { x[1] = (float)-sqrt(1 / 5.); x[2] = (float)sqrt(1 / 5.); }
+    if (n == 4) { x[1] = (float)-sqrt(15. / 35.); x[2] = 0.f; x[3] = (float)sqrt(15. / 35.); }
+    if (n == 5 || n == 6) {
+        const float a = n == 5 ? 315.f : 1386.f, c = n == 5 ? -210.f : -1260.f, e = n == 5 ? 15.f : 210.f;
+        const float disc = c * c - 4 * a * e;
+        const float r1 = (float)sqrt((-c + sqrt(disc)) / 2 / a), r2 = (float)sqrt((-c - sqrt(disc)) / 2 / a);
+        x[1] = -r1; x[2] = -r2;
+        if (n == 5) { x[3] = r2; x[4] = r1; }
+        else { x[3] = 0.f; x[4] = r2; x[5] = r1; }
+    }
+}
+
+/* Lagrange basis i of the nodes x at s, in float (lagrange.c:20-32) */
+static float lagrange_float(int n, int i, const float *x, float s)
+{
+    float l = 1.f;
+    for (int j = 0; j <= n; ++j)
+        if (j != i) l = l * (s - x[j]) / (x[i] - x[j]);
+    return l;
+}
+
+static FILE *open_out(const semsks_model *m, const char *name, char *path, size_t cap)
+{
+    snprintf(path, cap, "%s/%s", m->outdir[0] ? m->outdir : ".", name);
+    return fopen(path, "w");
+}
+
+semsks_status semsks_write_check(const semsks_model *m)
+{
+    if (!m) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
+    if (m->check != 1) return SEMSKS_OK;
+    const int nx = m->nx, nz = m->nz, hpn = m->prob.hpn;
+    const float h = m->prob.eh / hpn;
+    const float *T = m->mat_table;
+    const uint16_t *id = m->mat_id;
+    char path[2048];
+    /* grid.xz (sem2d_psvsh.c:249-260): rho on every second node column and
+       row, then, for each column, the top row's rho printed with the
+       coordinates of the last row written */
+    FILE *f = open_out(m, "grid.xz", path, sizeof path);
+    if (!f) return semsks_set_error(SEMSKS_EIO, "Error Opening file %s with access wt", path);
+    setvbuf(f, NULL, _IOFBF, 1 << 20);
+    for (int i = 0; i < nx; i += 2) {
+        float gx = 0.f, gz = 0.f;
+        for (int j = 0; j < nz; j += 2) {
+            gx = i * h;
+            gz = (nz - 1 - j) * h;
+            fprintf(f, "%-8.2f %-9.2f %9.4f\n", gx, gz, T[(size_t)id[(size_t)j * nx + i] * SEMSKS_MAT_STRIDE]);
+        }
+        fprintf(f, "%-8.2f %-9.2f %9.4f\n", gx, gz, T[(size_t)id[(size_t)(nz - 1) * nx + i] * SEMSKS_MAT_STRIDE]);
+    }
+    if (fclose(f) != 0) return semsks_set_error(SEMSKS_EIO, "short write on %s", path);
+    /* legendre.xz (:262-270): each basis function on [-1, 1] in steps of
+       0.05 (a float sum, as the reference's loop variable) */
+    f = open_out(m, "legendre.xz", path, sizeof path);
+    if (!f) return semsks_set_error(SEMSKS_EIO, "Error Opening file %s with access wt", path);
+    float x[8];
+    gll_float(hpn, x);
+    for (int i = 0; i <= hpn; ++i) {
+        fprintf(f, "> %-9.5f\n", x[i]);
+        for (float lx = -1; lx <= 1.0; lx = (float)(lx + 0.05))
+            fprintf(f, "%-8.2f %-9.3f\n", lx, lagrange_float(hpn, i, x, lx));
+    }
+    if (fclose(f) != 0) return semsks_set_error(SEMSKS_EIO, "short write on %s", path);
+    /* vel.xy (:804-825): per node, x in degrees of epicentral distance,
+       depth, and vp = sqrt(c11/rho) or, with output_vd, the shear-wave
+       splitting (v44 - v55)/(v44 + v55)/2 */
+    f = open_out(m, "vel.xy", path, sizeof path);
+    if (!f) return semsks_set_error(SEMSKS_EIO, "Error Opening file %s with access w", path);
+    setvbuf(f, NULL, _IOFBF, 1 << 20);
+    const double deg = 6371 * 3.14159265 / 180.;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < nz; ++j) {
+            const float *r = T + (size_t)id[(size_t)j * nx + i] * SEMSKS_MAT_STRIDE;
+            float v;
+            if (m->output_vd == 0) {
+                v = (float)sqrt(r[1] / r[0]);
+            } else {
+                const float v44 = (float)sqrt(r[10] / r[0]), v55 = (float)sqrt(r[13] / r[0]);
+                v = (float)((v44 - v55) / (v44 + v55) / 2.);
+            }
+            fprintf(f, "%-8.3f %-8.2f %-8.2f\n", (m->xmin + i * h) / deg, (nz - 1 - j) * h, v);
+        }
+    if (fclose(f) != 0) return semsks_set_error(SEMSKS_EIO, "short write on %s", path);
+    return SEMSKS_OK;
+}
+

@@ -26,7 +26,7 @@ STATUS = {0: "OK", 1: "EINVAL", 2: "EIO", 3: "ENOHBM", 4: "ENONFINITE", 5: "ERAN
 
 EXPORTS = [
     "semsks_load_par", "semsks_model_problem", "semsks_model_dims", "semsks_model_outdir",
-    "semsks_model_free", "semsks_write_seis", "semsks_last_error", "semsks_engine_create",
+    "semsks_model_free", "semsks_write_check", "semsks_write_seis", "semsks_last_error", "semsks_engine_create",
     "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_admission_bytes", "semsks_advance", "semsks_prepare",
     "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_retire", "semsks_sync",
     "semsks_last_timing", "semsks_last_timed_steps", "semsks_sweep_steps", "semsks_tile_rows", "semsks_run_problem", "semsks_stiff", "semsks_stiff_dev",
@@ -96,6 +96,7 @@ def lib() -> C.CDLL:
     L.semsks_model_outdir.restype = C.c_char_p
     L.semsks_model_outdir.argtypes = [vp]
     L.semsks_model_free.argtypes = [vp]
+    L.semsks_write_check.argtypes = [vp]
     L.semsks_write_seis.argtypes = [C.c_char_p, C.c_int, C.c_int, vp]
     L.semsks_engine_create.argtypes = [C.POINTER(EngineOpts), C.POINTER(vp)]
     L.semsks_engine_destroy.argtypes = [vp]
@@ -160,6 +161,11 @@ class Model:
         tab = np.ctypeslib.as_array(p.mat_table, shape=(p.n_mat * MAT_STRIDE,)).reshape(p.n_mat, MAT_STRIDE)
         ids = np.ctypeslib.as_array(p.mat_id, shape=(self.nx * self.nz,))
         return tab[ids]
+
+    def write_check(self):
+        """grid.xz, legendre.xz and vel.xy into the model's outdir when the
+        par file leaves check at 1 (semsks_write_check)."""
+        _check(lib().semsks_write_check(self._h))
 
     def receivers(self):
         p = self.problem

@@ -127,8 +127,14 @@ def test_cli_dropin(small):
     print("case: small, CLI", flush=True)
     os.makedirs("gpu", exist_ok=True)
     os.makedirs("cpu", exist_ok=True)
-    r = subprocess.run([semsks.CLI_PATH, "par=run.par", "outdir=gpu"], capture_output=True, text=True, timeout=300)
+    r = subprocess.run([semsks.CLI_PATH, "par=run.par", "outdir=gpu", "check=1"], capture_output=True, text=True,
+                       timeout=300)
     assert r.returncode == 0, r.stderr
+    # check=1: the setup diagnostics go to outdir before the solve
+    m = semsks.Model(["par=run.par"])
+    assert sum(1 for _ in open("gpu/vel.xy")) == m.nx * m.nz
+    assert sum(1 for _ in open("gpu/legendre.xz")) == 5 * 41
+    assert sum(1 for _ in open("gpu/grid.xz")) == len(range(0, m.nx, 2)) * (len(range(0, m.nz, 2)) + 1)
     assert pyoracle.main(["par=run.par"], "cpu", kind="f64") == 0
     a = np.stack([np.fromfile("gpu/seis." + c, np.float32) for c in ("u", "w", "v")], axis=1)
     b = np.stack([np.fromfile("cpu/seis." + c, np.float32) for c in ("u", "w", "v")], axis=1)
