@@ -126,6 +126,18 @@ void semsks_model_free(semsks_model *m);
    Written into the model's outdir before the solve; nothing when check=0. */
 semsks_status semsks_write_check(const semsks_model *m);
 
+/* The snapshot keys (sem2d_psvsh.c:149-150,202-204): k[0] plot_snap
+   (default 0), k[1] startsnap (100), k[2] incresnap (50).  Snapshots
+   out1..out26 are the levels startsnap + j incresnap, j = 0..25, that the
+   run reaches (:483-510). */
+void semsks_model_snapshots(const semsks_model *m, int k[3]);
+
+/* Writes one snapshot file (snapshot(), sem2d_psvsh.c:1699-1715): for every
+   second node column ix and row iz, "ix iz u v" (%d %d %e %e) from a field
+   in semsks_get_field's layout [3][nz][nx] (components x, y, z); the
+   reference prints components 0 and 1 in every solver. */
+semsks_status semsks_write_snapshot(const char *outdir, const char *name, int nx, int nz, const float *field);
+
 /* Writes seis.u / seis.w / seis.v (nst records of ntrace float32 each) from
    a trace block [nst][3][ntrace] ordered u, -w, v (kirrecord_zhl,
    sem2d_psvsh.c:2116-2165). */
@@ -214,6 +226,13 @@ semsks_status semsks_get_traces(semsks_engine *e, int slot, int it0, int n, floa
    interface row and column already hold the injection of the next step
    (GRT_Interface4/3 act on u(t) in place, after its traces are recorded). */
 semsks_status semsks_get_field(semsks_engine *e, int slot, float *out);
+
+/* The slot's current level as the reference's loop leaves it at the end of
+   an iteration, before the next step's interface injection: semsks_get_field
+   with the interface row and column restored to their values before that
+   injection (the field snapshot() writes, sem2d_psvsh.c:483-510,1699-1715).
+   Same layout as semsks_get_field. */
+semsks_status semsks_get_snapshot(semsks_engine *e, int slot, float *out);
 
 /* Frees a slot (its device memory returns to the budget). */
 semsks_status semsks_retire(semsks_engine *e, int slot);

@@ -7,6 +7,7 @@
  */
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include "semsks.h"
 
 static int exit_code(semsks_status st)
@@ -20,6 +21,52 @@ static int exit_code(semsks_status st)
     case SEMSKS_ESTOP: return 101;
     default: return 1;
     }
+}
+
+/* The run with plot_snap=1: the engine stepped from snapshot level to
+   snapshot level (startsnap + j incresnap, j = 0..25, the levels the run
+   reaches, sem2d_psvsh.c:483-510), each written as out<j+1> before the
+   rest of the run; the traces as semsks_run_problem gives them. */
+static semsks_status run_with_snapshots(const semsks_model *m, const semsks_problem *p, int nx, int nz, float *tr)
+{
+    int k[3];
+    semsks_model_snapshots(m, k);
+    long lv[26];
+    int idx[26], n = 0;
+    for (int j = 0; j < 26; ++j) {
+        const long L = (long)k[1] + (long)j * k[2];
+        if (L < 0 || L > p->nst - 1) continue;
+        int i = n++;
+        for (; i > 0 && lv[i - 1] > L; --i) { lv[i] = lv[i - 1]; idx[i] = idx[i - 1]; }
+        lv[i] = L;
+        idx[i] = j;
+    }
+    semsks_engine_opts o;
+    memset(&o, 0, sizeof o);
+    o.max_slots = 1;
+    o.use_graph = 1;
+    semsks_engine *e = NULL;
+    semsks_status st = semsks_engine_create(&o, &e);
+    if (st != SEMSKS_OK) return st;
+    int slot = -1;
+    long level = 0;
+    float *field = (float *)malloc(sizeof(float) * 3 * (size_t)nx * (size_t)nz);
+    st = field ? semsks_admit(e, p, -1, &slot) : SEMSKS_EINVAL;
+    for (int i = 0; i < n && st == SEMSKS_OK; ++i) {
+        if (lv[i] > level) {
+            st = semsks_advance(e, (int)(lv[i] - level), 1);
+            level = lv[i];
+        }
+        char name[16];
+        snprintf(name, sizeof name, "out%d", idx[i] + 1);
+        if (st == SEMSKS_OK) st = semsks_get_snapshot(e, slot, field);
+        if (st == SEMSKS_OK) st = semsks_write_snapshot(semsks_model_outdir(m), name, nx, nz, field);
+    }
+    if (st == SEMSKS_OK && level < p->nst - 1) st = semsks_advance(e, (int)(p->nst - 1 - level), 1);
+    if (st == SEMSKS_OK) st = semsks_get_traces(e, slot, 0, p->nst, tr);
+    free(field);
+    semsks_engine_destroy(e);
+    return st;
 }
 
 int main(int argc, char **argv)
@@ -45,7 +92,9 @@ int main(int argc, char **argv)
     }
     size_t n = (size_t)p->nst * 3 * (p->ntrace > 0 ? p->ntrace : 1);
     float *tr = (float *)calloc(n, sizeof(float));
-    st = semsks_run_problem(p, tr);
+    int snap[3];
+    semsks_model_snapshots(m, snap);
+    st = snap[0] ? run_with_snapshots(m, p, dims[0], dims[1], tr) : semsks_run_problem(p, tr);
     /* an unstable run still writes its traces, NaN/Inf included, as the
        reference does (its fwrite in kirrecord_zhl, sem2d_psvsh.c:2116-2165,
        never checks the values): a later uvw2sac step then converts this

@@ -469,6 +469,7 @@ __device__ __forceinline__ void inject_node(const InstDesc &D, int it, int i, in
     float d[3];
     inject_delta(D, it, ix, iz, mp ? Up[3 * q] : 0.f, mp ? Up[3 * q + 2] : 0.f, d);
     float u[3] = {U[3 * q], U[3 * q + 1], U[3 * q + 2]};
+    for (int k = 0; k < 3; ++k) D.pre_inj[3 * i + k] = u[k];
     uint32_t w = D.comp[p][q];
     kahan3(D.kahan != 0, u, w, d);
     for (int k = 0; k < 3; ++k) U[3 * q + k] = u[k];
@@ -2029,6 +2030,7 @@ size_t semsks_footprint(const semsks_problem *p, int shared)
     int nlev = std::min(std::max(nt_sem_of(p), 1), std::max(p->nst, 1));
     b += (size_t)nlev * (mt + sht) * 4;
     b += (size_t)p->nst * 3 * std::max(p->ntrace, 1) * 4 + (size_t)p->ntrace * 8;
+    b += (size_t)(nx_of(p) + nz_of(p)) * 12;        /* pre_inj (bound) */
     return b;
 }
 
@@ -2395,6 +2397,9 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
         int *dhead = nullptr, *dnext = nullptr;
         ACHK(dalloc(s, &dhead, head.size()));
         ACHK(dalloc(s, &dnext, next.size()));
+        /* the interface nodes of the newest level before its injection
+           (semsks_get_snapshot) */
+        ACHK(dalloc(s, &d.pre_inj, (size_t)std::max(nb + nl, 1) * 3));
         ACHK(hipMemcpyAsync(dhead, head.data(), head.size() * 4, hipMemcpyHostToDevice, e->setup));
         ACHK(hipMemcpyAsync(dnext, next.data(), next.size() * 4, hipMemcpyHostToDevice, e->setup));
         d.inj_rec = dhead; d.rec_next = dnext;
@@ -2646,6 +2651,25 @@ semsks_status semsks_get_field(semsks_engine *e, int k, float *out)
         for (int iz = 0; iz < s.nz; ++iz)
             for (int ix = 0; ix < s.nx; ++ix)
                 out[((size_t)c * s.nz + iz) * s.nx + ix] = aos[node_at(s.bw, s.nz, ix, iz) * 3 + c];
+    return SEMSKS_OK;
+}
+
+semsks_status semsks_get_snapshot(semsks_engine *e, int k, float *out)
+{
+    semsks_status st = semsks_get_field(e, k, out);
+    if (st != SEMSKS_OK) return st;
+    const Slot &s = e->slots[k];
+    const int it = e->h_levels[k];
+    /* injects(): the current level holds the next step's injection */
+    if (!(it < s.meta.nst - 1 && it + 1 < s.nt_sem)) return SEMSKS_OK;
+    const int nbl = s.meta.neh * s.meta.hpn, nb = s.nx - nbl, nl = s.nz - nbl - 1;
+    std::vector<float> pre((size_t)(nb + nl) * 3);
+    HIPCHK(hipMemcpyAsync(pre.data(), s.d.pre_inj, pre.size() * 4, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int i = 0; i < nb + nl; ++i) {
+        const int ix = i < nb ? nbl + i : nbl, iz = i < nb ? nbl : nbl + 1 + (i - nb);
+        for (int c = 0; c < 3; ++c) out[((size_t)c * s.nz + iz) * s.nx + ix] = pre[3 * i + c];
+    }
     return SEMSKS_OK;
 }
 

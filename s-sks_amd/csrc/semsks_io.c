@@ -223,6 +223,7 @@ struct semsks_model {
     semsks_problem prob;
     int nx, nz, nt_sem, plot_trace;
     int check, output_vd;           /* check=1 (the default): the setup diagnostics files */
+    int plot_snap, startsnap, incresnap;
     float xmin;                     /* epicentral distance of the mesh's x = 0, km */
     char outdir[1024];
     float *mat_table;
@@ -544,6 +545,11 @@ semsks_status semsks_load_par(int argc, char **argv, semsks_model **out)
     M->check = 1;                   /* sem2d_psvsh.c:47,179 */
     get_i(&P, "check", &M->check);
     get_i(&P, "output_vd", &M->output_vd);
+    M->startsnap = 100;             /* sem2d_psvsh.c:149-150,202-204 */
+    M->incresnap = 50;
+    get_i(&P, "startsnap", &M->startsnap);
+    get_i(&P, "incresnap", &M->incresnap);
+    get_i(&P, "plot_snap", &M->plot_snap);
     if (get_rec == 1) req_s(&P, "rec_file", rec_file, sizeof rec_file);
     req_i(&P, "idxplot", &idxplot);
     req_i(&P, "msource", &msource);
@@ -747,6 +753,29 @@ semsks_status semsks_write_check(const semsks_model *m)
                 v = (float)((v44 - v55) / (v44 + v55) / 2.);
             }
             fprintf(f, "%-8.3f %-8.2f %-8.2f\n", (m->xmin + i * h) / deg, (nz - 1 - j) * h, v);
+        }
+    if (fclose(f) != 0) return semsks_set_error(SEMSKS_EIO, "short write on %s", path);
+    return SEMSKS_OK;
+}
+
+void semsks_model_snapshots(const semsks_model *m, int k[3])
+{
+    k[0] = m->plot_snap; k[1] = m->startsnap; k[2] = m->incresnap;
+}
+
+semsks_status semsks_write_snapshot(const char *outdir, const char *name, int nx, int nz, const float *u)
+{
+    if (!name || !u || nx < 1 || nz < 1) return semsks_set_error(SEMSKS_EINVAL, "bad arguments");
+    char path[2048];
+    snprintf(path, sizeof path, "%s/%s", outdir && *outdir ? outdir : ".", name);
+    FILE *f = fopen(path, "wt");
+    if (!f) return semsks_set_error(SEMSKS_EIO, "Error Opening file %s with access wt", path);
+    setvbuf(f, NULL, _IOFBF, 1 << 20);
+    const size_t plane = (size_t)nx * nz;
+    for (int ix = 0; ix < nx; ix += 2)
+        for (int iz = 0; iz < nz; iz += 2) {
+            const size_t q = (size_t)iz * nx + ix;
+            fprintf(f, "%d %d %e %e\n", ix, iz, u[q], u[plane + q]);
         }
     if (fclose(f) != 0) return semsks_set_error(SEMSKS_EIO, "short write on %s", path);
     return SEMSKS_OK;

@@ -62,6 +62,10 @@ struct InstDesc {
     const int *rec_next;            /* per receiver: next one on its interface node (-1 at the
                                        end), -2 if it is not on an interface node */
     const float *taper;             /* sponge factors [nh] (atten > 0) */
+    /* values of the interface nodes (inject_node's order) of the newest
+       level before its injection, [n_inject][3]: the field as the
+       reference's loop leaves it at the end of an iteration */
+    float *pre_inj;
     int atten, nh;
     float *traces;
     /* element operators (FP32 from FP64):

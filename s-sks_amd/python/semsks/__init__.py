@@ -26,9 +26,11 @@ STATUS = {0: "OK", 1: "EINVAL", 2: "EIO", 3: "ENOHBM", 4: "ENONFINITE", 5: "ERAN
 
 EXPORTS = [
     "semsks_load_par", "semsks_model_problem", "semsks_model_dims", "semsks_model_outdir",
-    "semsks_model_free", "semsks_write_check", "semsks_write_seis", "semsks_last_error", "semsks_engine_create",
+    "semsks_model_free", "semsks_write_check", "semsks_model_snapshots", "semsks_write_snapshot",
+    "semsks_write_seis", "semsks_last_error", "semsks_engine_create",
     "semsks_engine_destroy", "semsks_admit", "semsks_footprint", "semsks_admission_bytes", "semsks_advance", "semsks_prepare",
-    "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_retire", "semsks_sync",
+    "semsks_slot_status", "semsks_get_traces", "semsks_get_field", "semsks_get_snapshot", "semsks_retire",
+    "semsks_sync",
     "semsks_last_timing", "semsks_last_timed_steps", "semsks_sweep_steps", "semsks_tile_rows", "semsks_run_problem", "semsks_stiff", "semsks_stiff_dev",
     "semsks_get_setup", "semsks_get_boundary_series", "semsks_planewave", "semsks_serve", "semsks_serve_ex",
     "semsks_uvw2sac",
@@ -97,6 +99,8 @@ def lib() -> C.CDLL:
     L.semsks_model_outdir.argtypes = [vp]
     L.semsks_model_free.argtypes = [vp]
     L.semsks_write_check.argtypes = [vp]
+    L.semsks_model_snapshots.argtypes = [vp, C.POINTER(C.c_int)]
+    L.semsks_write_snapshot.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, vp]
     L.semsks_write_seis.argtypes = [C.c_char_p, C.c_int, C.c_int, vp]
     L.semsks_engine_create.argtypes = [C.POINTER(EngineOpts), C.POINTER(vp)]
     L.semsks_engine_destroy.argtypes = [vp]
@@ -110,6 +114,7 @@ def lib() -> C.CDLL:
     L.semsks_slot_status.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.semsks_get_traces.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp]
     L.semsks_get_field.argtypes = [vp, C.c_int, vp]
+    L.semsks_get_snapshot.argtypes = [vp, C.c_int, vp]
     L.semsks_retire.argtypes = [vp, C.c_int]
     L.semsks_sync.argtypes = [vp]
     L.semsks_last_timing.argtypes = [vp, C.POINTER(C.c_double)]
@@ -166,6 +171,12 @@ class Model:
         """grid.xz, legendre.xz and vel.xy into the model's outdir when the
         par file leaves check at 1 (semsks_write_check)."""
         _check(lib().semsks_write_check(self._h))
+
+    def snapshots(self):
+        """(plot_snap, startsnap, incresnap) from the par keys."""
+        k = (C.c_int * 3)()
+        lib().semsks_model_snapshots(self._h, k)
+        return tuple(k)
 
     def receivers(self):
         p = self.problem
@@ -255,6 +266,15 @@ class Engine:
         nx, nz = p.nex * p.hpn + 1, p.nez * p.hpn + 1
         out = np.zeros((3, nz, nx), dtype=np.float32)
         _check(lib().semsks_get_field(self._h, slot, out.ctypes.data))
+        return out
+
+    def snapshot(self, slot: int) -> np.ndarray:
+        """The current level before the next step's interface injection
+        (semsks_get_snapshot), [3][nz][nx]."""
+        p = self.slots[slot]
+        nx, nz = p.nex * p.hpn + 1, p.nez * p.hpn + 1
+        out = np.zeros((3, nz, nx), dtype=np.float32)
+        _check(lib().semsks_get_snapshot(self._h, slot, out.ctypes.data))
         return out
 
     def stiff(self, slot: int, u: np.ndarray) -> np.ndarray:

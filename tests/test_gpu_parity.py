@@ -143,6 +143,53 @@ def test_cli_dropin(small):
         assert l2 <= REL_L2_TOL and pk <= PEAK_TOL, (k, l2, pk)
 
 
+def test_snapshots_vs_fp64(tmp_path, monkeypatch):
+    """semsks_get_snapshot at a level is the FP64 oracle's field at the end
+    of that iteration (before the next step's interface injection, which
+    the field of the engine already holds); the CLI with plot_snap=1 writes
+    it as out<k> (sem2d_psvsh.c:483-510,1699-1715)."""
+    print("case: small, snapshots", flush=True)
+    d = make_case(tmp_path, solver=0, sh_zero=False, psi_deg=30.0)
+    monkeypatch.chdir(d)
+    m = semsks.Model(["par=run.par"])
+    p = m.problem
+    o64 = pyoracle.Oracle(["par=run.par"], "f64")
+    nbl = p.neh * p.hpn
+    iz, ix = np.meshgrid(np.arange(m.nz), np.arange(m.nx), indexing="ij")
+    iface = ((iz == nbl) & (ix >= nbl)) | ((ix == nbl) & (iz > nbl))
+    pending = 0.0
+    snaps = {}
+    with semsks.Engine() as e:
+        s = e.admit(p)
+        lev = 0
+        for L in (0, 150, 300, p.nst - 1):
+            e.advance(L - lev)
+            lev = L
+            snap, field = e.snapshot(s), e.field(s)
+            snaps[L] = snap
+            _, ref = o64.run(L + 1, want_field=True)
+            assert np.array_equal(snap[:, ~iface], field[:, ~iface])
+            scale = np.abs(ref).max()
+            if scale == 0:                    # level 0 before the pulse arrives
+                assert np.abs(snap).max() == 0
+                continue
+            err = np.abs(snap - ref).max()
+            assert err <= 1e-4 * scale, (L, err, scale)
+            pending = max(pending, np.abs(field - ref)[:, iface].max() / scale)
+    # at some level the injection the field holds is far above the bar:
+    # restoring the interface nodes is what makes the snapshot match
+    assert pending > 1e-2, pending
+    os.makedirs("gpu", exist_ok=True)
+    r = subprocess.run([semsks.CLI_PATH, "par=run.par", "outdir=gpu", "plot_snap=1", "startsnap=0", "incresnap=150"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert [os.path.exists("gpu/out%d" % k) for k in range(1, 6)] == [True, True, True, True, False]
+    rows = np.loadtxt("gpu/out2")
+    sel = (slice(None, None, 2), slice(None, None, 2))
+    want = np.stack([snaps[150][0][sel].T.ravel(), snaps[150][1][sel].T.ravel()], axis=1)
+    assert np.allclose(rows[:, 2:], want, rtol=5.1e-7, atol=1e-30)
+
+
 def test_batch_members_match_solo(tmp_path, monkeypatch):
     """Instances stepped together in one launch give the same traces as alone."""
     print("case: small, batch of 3 vs solo", flush=True)
