@@ -126,18 +126,23 @@ def test_c2_full_length_stays_finite():
     assert 1e-4 * peak[0] < peak[2] < 0.1 * peak[0], peak
 
 
-def test_c2_full_mesh_field_vs_fp64(tmp_path, monkeypatch):
+@pytest.mark.parametrize("name", ["c2", "c3"])
+def test_full_mesh_field_vs_fp64(tmp_path, monkeypatch, name):
     """The production mesh against the FP64 oracle: configs.c2_case (2000x500
-    elements, tilted-axis upper mantle, coupled solver) with the plane S wave
-    30 degrees off SV, written in the reference's file formats, stepped to
-    level 2000; the field as the reference's loop leaves it
-    (semsks_get_snapshot) equals the oracle's at every 20th node column and
-    row within 1e-4 of each component's peak (fixture
-    tests/golden/c2_fullsize_fp64_field.npz, by make_c2_fullsize_fp64.py)."""
-    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_fullsize_fp64_field.npz"))
+    elements, tilted-axis upper mantle: the general stress form) and
+    configs.c3_case (the batch's isotropic layers: the short orthotropic
+    form), coupled solver, a plane S wave 30 degrees off SV, written in the
+    reference's file formats, stepped to level 2000; the field as the
+    reference's loop leaves it (semsks_get_snapshot) equals the oracle's at
+    every 20th node column and row within 1e-4 of each component's peak
+    (fixtures tests/golden/<name>_fullsize_fp64_field.npz, by
+    make_fullsize_fp64.py)."""
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "%s_fullsize_fp64_field.npz" % name))
     nst, stride, ref = int(z["nst"]), int(z["stride"]), z["field"]
-    print("case: c2 full mesh 2000x500, %d steps vs FP64 field" % (nst - 1), flush=True)
-    configs.c2_case(nst=nst, psi_deg=30.0, sh_zero=False).write(os.fspath(tmp_path))
+    print("case: %s full mesh 2000x500, %d steps vs FP64 field" % (name, nst - 1), flush=True)
+    case = configs.c2_case(nst=nst, psi_deg=30.0, sh_zero=False) if name == "c2" else \
+        configs.c3_case(nst=nst, p=0.06, psi_deg=30.0)
+    case.write(os.fspath(tmp_path))
     monkeypatch.chdir(tmp_path)
     m = semsks.Model(["par=run.par"])
     with semsks.Engine(max_slots=1) as e:
