@@ -536,8 +536,8 @@ def main():
             c4 = run_c4(args.steps, args.warmup, tile_z=args.tile_z, kernel_steps=args.kernel_steps)
             c4["traffic_bytes_per_launch"] = profiled_traffic("step_kernel_uneven96_c4.md")
             if c4["traffic_bytes_per_launch"]:
-                # the reference's bytes price per-node constants this build does not
-                # read (frac above 1): the measured traffic is the honest roofline
+                # the reference's bytes price per-node constants the step kernel does
+                # not read (frac above 1): the measured traffic is the honest roofline
                 c4["traffic_GBs"] = c4["traffic_bytes_per_launch"] / (c4["kernel_ms"] * 1e-3) / 1e9
                 c4["frac_measured_traffic"] = c4["traffic_GBs"] / HBM_PEAK_GBS
             c4["frac_alg_bytes"] = c4["achieved_GBs"] / HBM_PEAK_GBS

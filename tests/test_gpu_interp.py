@@ -6,7 +6,7 @@ sem2d_psvsh.c:388), the coupled solver with one slot ring shared by its
 P-SV and SH streams (sem2d_psvsh.c:430-431), then repairs non-finite and
 |g| >= 10 values in place, left to right (sem2d_psvsh.c:433-443).  At the
 two ends of a vector the reference reads g[-1] and g[mtotal], outside the
-array; this build reads 0 there, as the oracle does.
+array; the device path reads 0 there, as the oracle does.
 
 Bars (bit-exact): the device series of every level against the vectors of
 the reference's own compiled interpl.c (tests/golden/ref_interpl_mesh.npz,
