@@ -21,8 +21,8 @@ F32 = np.float32
 DEG = 6371 * 3.14159265 / 180.0
 
 
-def model_in(tmp_path, monkeypatch, **extra):
-    c = inputs.small_case(extra=dict(check=1, xmin="250.0", **extra))
+def model_in(tmp_path, monkeypatch, hpn=4, **extra):
+    c = inputs.small_case(hpn=hpn, extra=dict(check=1, xmin="250.0", **extra))
     d = os.fspath(tmp_path)
     D = c.write(d)
     monkeypatch.chdir(d)
@@ -55,11 +55,19 @@ def test_check_zero_writes_nothing_and_the_default_is_one(tmp_path, monkeypatch)
         assert os.path.getsize(f) > 0
 
 
-def test_legendre_basis_and_gll_nodes(tmp_path, monkeypatch):
-    m, D, _ = model_in(tmp_path, monkeypatch)
+def gll_nodes(n):
+    """the n+1 Gauss-Lobatto-Legendre nodes: the ends and the roots of P_n'"""
+    inner = np.polynomial.legendre.Legendre.basis(n).deriv().roots()
+    return np.concatenate([[-1.0], np.sort(inner.real), [1.0]])
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6])
+def test_legendre_basis_and_gll_nodes(tmp_path, monkeypatch, n):
+    m, D, _ = model_in(tmp_path, monkeypatch, hpn=n)
     m.write_check()
-    n = 4
-    gll = np.array([-1.0, -np.sqrt(3.0 / 7.0), 0.0, np.sqrt(3.0 / 7.0), 1.0])
+    gll = gll_nodes(n)
+    if n == 4:
+        assert np.allclose(gll, [-1.0, -np.sqrt(3.0 / 7.0), 0.0, np.sqrt(3.0 / 7.0), 1.0])
     blocks = []
     for line in open("legendre.xz"):
         if line.startswith(">"):
@@ -85,7 +93,7 @@ def test_legendre_basis_and_gll_nodes(tmp_path, monkeypatch):
     # partition of unity, and the Kronecker property at x = -1
     for k in range(len(xs)):
         assert abs(sum(b[1][k][1] for b in blocks) - 1.0) <= (n + 1) * 5.1e-4
-    assert np.allclose([b[1][0][1] for b in blocks], [1, 0, 0, 0, 0], atol=5.1e-4)
+    assert np.allclose([b[1][0][1] for b in blocks], np.eye(n + 1)[0], atol=5.1e-4)
 
 
 def test_grid_densities_by_layer(tmp_path, monkeypatch):
