@@ -51,7 +51,12 @@ static semsks_status run_with_snapshots(const semsks_model *m, const semsks_prob
     int slot = -1;
     long level = 0;
     float *field = (float *)malloc(sizeof(float) * 3 * (size_t)nx * (size_t)nz);
-    st = field ? semsks_admit(e, p, -1, &slot) : SEMSKS_EINVAL;
+    if (!field) {
+        fprintf(stderr, "semsks: out of host memory for a %d x %d snapshot\n", nx, nz);
+        semsks_engine_destroy(e);
+        return SEMSKS_EINVAL;
+    }
+    st = semsks_admit(e, p, -1, &slot);
     for (int i = 0; i < n && st == SEMSKS_OK; ++i) {
         if (lv[i] > level) {
             st = semsks_advance(e, (int)(lv[i] - level), 1);
