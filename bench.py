@@ -526,6 +526,9 @@ def main():
                "c1_mesh_tile_rows_plain": r1p["tile_rows"], "c1_mesh_tile_rows_batched": r1b["tile_rows"],
                "c1_mesh_batched_over_plain": r1b["throughput"] / r1p["throughput"],
                "c1_mesh_workload": "64 plane S waves on the 200x100-element mesh vs one",
+               "c1_mesh_plain_note": "one instance's state (three levels, about 16 MB) fits in the 256 MB "
+                                     "Infinity Cache: the plain run on this mesh is cache-resident and "
+                                     "launch-bound, the batches of 64-1024 waves (1-16 GB) are not",
                "c1_mesh_batched256_value": r1c["throughput"], "c1_mesh_tile_rows_batched256": r1c["tile_rows"],
                "c1_mesh_batched256_over_plain": r1c["throughput"] / r1p["throughput"],
                "c1_mesh_batched1024_value": r1d["throughput"], "c1_mesh_tile_rows_batched1024": r1d["tile_rows"],
