@@ -16,7 +16,7 @@ import pytest
 
 import pyoracle
 import semsks
-from semsks import inputs
+from semsks import configs, inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -192,6 +192,30 @@ def test_snapshots_vs_fp64(tmp_path, monkeypatch, solver):
     sel = (slice(None, None, 2), slice(None, None, 2))
     want = np.stack([snaps[150][0][sel].T.ravel(), snaps[150][1][sel].T.ravel()], axis=1)
     assert np.allclose(rows[:, 2:], want, rtol=5.1e-7, atol=1e-30)
+
+
+def test_device_planewave_matches_the_greenfile_writer():
+    """semsks_planewave, the device generator of the batch and serving paths,
+    against inputs.greenfile_records, which writes the greenfile of every
+    file-based case and FP64 fixture: the same closed-form plane wave in
+    FP64 at the interface nodes, rounded to float, in the greenfile layout
+    (u, w down along the bottom row, then the left column; v in the SH
+    file).  Within a few ulps of the peak (the device's and numpy's exp and
+    cos may differ in the last bit)."""
+    print("case: small, device plane-wave records vs the greenfile writer", flush=True)
+    c = inputs.small_case(sh_zero=False, psi_deg=30.0)
+    b = configs.Built(c)
+    D, nt = b.D, 400
+    with semsks.Engine() as e:
+        psv, sh = e.planewave(*b.planewave_args(nt), to_host=True)
+    pw = inputs.PlaneWave(p=c.p, beta=D["vs"][-1], f0=c.f0, psi_deg=c.psi_deg)
+    ref_psv, ref_sh = inputs.greenfile_records(pw, c.nex, D["nez"], D["eh"], c.hpn, c.neh, nt,
+                                               float(np.float32(D["sdt"])))
+    assert psv.shape == ref_psv.shape and sh.shape == ref_sh.shape
+    scale = np.abs(ref_psv).max()
+    assert scale > 0.5 and np.abs(ref_sh).max() > 0.2
+    assert np.abs(psv - ref_psv).max() <= 4e-7 * scale
+    assert np.abs(sh - ref_sh).max() <= 4e-7 * scale
 
 
 def test_batch_members_match_solo(tmp_path, monkeypatch):
