@@ -72,10 +72,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0):
+def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0, flags=0):
     """Admits one instance per (built, p, psi) entry (boundary records made by
     the device plane-wave generator), runs warmup + timed steps in hipGraph
-    mode.  Returns throughput (pt-steps/s), ms/step and the live engine."""
+    mode.  Returns throughput (pt-steps/s), ms/step and the live engine.
+    flags: semsks_problem flags of every instance (F_NO_KAHAN: the plain
+    FP32 update)."""
     import semsks
     nt = warmup + steps + extra_steps + 8
     eng = semsks.Engine(max_slots=len(waves), tile_z=tile_z, use_graph=1)
@@ -84,6 +86,7 @@ def run_engine(waves, steps, warmup, share_material, tile_z=0, extra_steps=0):
     for k, (b, p, psi) in enumerate(waves):
         prob = b.problem(nt)
         prob.nst = nt + 1
+        prob.flags = flags
         eng.planewave(*b.planewave_args(nt, p=p, psi=psi), to_host=False)
         share = slots[0] if (share_material and slots) else -1
         slots.append(eng.admit(prob, share=share))
@@ -373,6 +376,8 @@ def main():
     ap.add_argument("--no-c4", dest="c4", action="store_false", help="skip the uneven-batch (C4) record")
     ap.add_argument("--only", choices=["batched"], default=None,
                     help="c3: run only the batched leg (for profiler runs)")
+    ap.add_argument("--no-kahan", action="store_true",
+                    help="the batched leg with the plain FP32 update (for profiler runs with --only batched)")
     ap.add_argument("--kernel-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one device copy of this many MiB (PMC byte-counter calibration)")
@@ -429,13 +434,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.placement_spread:
+        import semsks
         from semsks import configs
         b3 = configs.Built(configs.c3_case(nst=10))
         waves = [(b3, p, baz) for p, baz in configs.c3_waves()][: args.batch]
         tz, res = args.tile_z, []
         for i in range(args.placement_spread):
             # the first engine autotunes when no height is given; the rest reuse its pick
-            r = run_engine(waves, 4, 4 if tz else 24, True, tile_z=tz, extra_steps=3 * args.kernel_steps)
+            r = run_engine(waves, 4, 4 if tz else 24, True, tile_z=tz, extra_steps=3 * args.kernel_steps,
+                           flags=semsks.F_NO_KAHAN if args.no_kahan else 0)
             tz = tz or r["tile_rows"]
             ks = [kernel_time(r["engine"], args.kernel_steps)[0] for _ in range(3)]
             r["engine"].close()
@@ -454,13 +461,23 @@ def main():
         log("batched: C3, %d instances" % args.batch)
         b3 = configs.Built(configs.c3_case(nst=10))
         waves = [(b3, p, baz) for p, baz in configs.c3_waves()][: args.batch]
-        r3 = run_engine(waves, args.steps, args.warmup, True, tile_z=args.tile_z, extra_steps=args.kernel_steps)
+        r3 = run_engine(waves, args.steps, args.warmup, True, tile_z=args.tile_z, extra_steps=args.kernel_steps,
+                        flags=semsks.F_NO_KAHAN if args.no_kahan else 0)
         k3, _ = kernel_time(r3["engine"], args.kernel_steps)
         r3["engine"].close()
         log("  batched %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms, tiles of %d rows" % (
             r3["throughput"], r3["ms_per_step"], k3, r3["tile_rows"]))
         if args.only == "batched":
             return
+        # the same batch with the plain FP32 update (the reference's own
+        # arithmetic): no compensation words read or written
+        log("batched, plain FP32 update (F_NO_KAHAN)")
+        r3f = run_engine(waves, args.steps, args.warmup, True, tile_z=args.tile_z, extra_steps=args.kernel_steps,
+                         flags=semsks.F_NO_KAHAN)
+        k3f, _ = kernel_time(r3f["engine"], args.kernel_steps)
+        r3f["engine"].close()
+        log("  batched, plain FP32 %.3e pt-steps/s, %.3f ms/step, kernel %.3f ms, tiles of %d rows" % (
+            r3f["throughput"], r3f["ms_per_step"], k3f, r3f["tile_rows"]))
         # plain, like for like: one instance of the same workload (same mesh,
         # model and kernels, batch 1)
         log("plain: 1 instance of the C3 workload")
@@ -518,6 +535,13 @@ def main():
                                           % (5 * rp["throughput"] * B / 1e12),
                "plain_hbm_bound": hbm_bound_plain,
                "kernel_ms_batched": k3, "kernel_ms_plain": kp,
+               "fp32_plain_batched_value": r3f["throughput"], "fp32_plain_batched_ms_per_step": r3f["ms_per_step"],
+               "kernel_ms_fp32_plain_batched": k3f,
+               "fp32_plain_batched_frac": B * r3["npts"] / (k3f * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               "fp32_plain_note": "the same batch with the plain FP32 update (F_NO_KAHAN, the reference's own "
+                                  "arithmetic): no compensation words read or written, 36 instead of 44 B of state "
+                                  "per point-step; its 20,000-step worst misfit against FP64 is about 4x the "
+                                  "default's (value is the default, compensated update)",
                "tile_rows_batched": r3["tile_rows"], "tile_rows_plain": rp["tile_rows"],
                "c2_workload": "1 instance, 2000x500 mesh, tilted-axis hexagonal 4% upper mantle (15 constants)",
                "c2_value": r2["throughput"], "c2_ms_per_step": r2["ms_per_step"], "kernel_ms_c2": k2,

@@ -470,10 +470,10 @@ __device__ __forceinline__ void inject_node(const InstDesc &D, int it, int i, in
     inject_delta(D, it, ix, iz, mp ? Up[3 * q] : 0.f, mp ? Up[3 * q + 2] : 0.f, d);
     float u[3] = {U[3 * q], U[3 * q + 1], U[3 * q + 2]};
     for (int k = 0; k < 3; ++k) D.pre_inj[3 * i + k] = u[k];
-    uint32_t w = D.comp[p][q];
+    uint32_t w = D.kahan ? D.comp[p][q] : 0u;
     kahan3(D.kahan != 0, u, w, d);
     for (int k = 0; k < 3; ++k) U[3 * q + k] = u[k];
-    D.comp[p][q] = w;
+    if (D.kahan) D.comp[p][q] = w;
 }
 
 /* does level it get an injection before the step that leaves it? */
@@ -525,7 +525,7 @@ __global__ void sponge_kernel(const InstDesc *insts, const int *levels)
         const long q = node_at(D, ix, iz);
         float *U1 = D.u[pn] + 3 * q;
         for (int k = 0; k < 3; ++k) U1[k] = f * U1[k];
-        D.comp[pn][q] &= ~C_OWN;          /* the previous level's fields stay */
+        if (D.kahan) D.comp[pn][q] &= ~C_OWN;          /* the previous level's fields stay */
     }
 }
 
@@ -774,13 +774,13 @@ __device__ __forceinline__ void update_one(const InstDesc &D, int p, long q, int
     const int e = edge_index(ix, iz, D.nx, D.nz, D.hpn);
     float up[3];
     xyz_load(gp(D.u[(p + D.nlev - 1) % D.nlev]), q, up);
-    const uint32_t w = gp(D.comp[p])[q];
+    const uint32_t w = D.kahan ? gp(D.comp[p])[q] : 0u;
     const float m = e < 0 ? mdt2_of<N>(D, mid, ix, iz) : 0.f;
     float nu[3];
     uint32_t wn;
     next_node(D, D.kahan != 0, e, m, f, uc, w, up, nu, wn);
     xyz_store(gp(D.u[(p + 1) % D.nlev]), q, nu);
-    gp(D.comp[(p + 1) % D.nlev])[q] = wn;
+    if (D.kahan) gp(D.comp[(p + 1) % D.nlev])[q] = wn;
 }
 
 /* Lane layout of the element-force/update kernel: a 64-lane wave holds
@@ -1027,7 +1027,7 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
             const bool vec = own && iz != 0 && ex != 0;
             r.m[0] = r.m[1] = r.m[2] = r.m[3] = 0.f;
             uint4 w4 = make_uint4(0u, 0u, 0u, 0u);
-            if (vec) w4 = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(gp(D.comp[p]) + rowoff);
+            if (vec && kahan) w4 = *reinterpret_cast<const SEMSKS_GMEM uint4 *>(gp(D.comp[p]) + rowoff);
             r.w[0] = w4.x; r.w[1] = w4.y; r.w[2] = w4.z; r.w[3] = w4.w;
             row_load<4>(gp(D.u[pp]) + 3 * rowoff, vec, r.pv);
             if (vec) {
@@ -1175,7 +1175,8 @@ __device__ void process_tile(const InstDesc &D, int p, int x0, int z0, int rows,
                     advance3(kahan, false, nv[a], nw[a], cur.pv[a], f[a], cur.m[a], one, zero);
                 }
                 row_store4(gp(D.u[pn]) + 3 * q0, nv);
-                *reinterpret_cast<SEMSKS_GMEM uint4 *>(gp(D.comp[pn]) + q0) = make_uint4(nw[0], nw[1], nw[2], nw[3]);
+                if (kahan)
+                    *reinterpret_cast<SEMSKS_GMEM uint4 *>(gp(D.comp[pn]) + q0) = make_uint4(nw[0], nw[1], nw[2], nw[3]);
                 if (last_x) {
                     float fa[3] = {f[N][0], f[N][1], f[N][2]};
                     float ua[3] = {u[N][0], u[N][1], u[N][2]};

@@ -59,10 +59,13 @@ def test_c1_compensation_reduces_the_error(tmp_path, monkeypatch):
     """The packed compensation (5-bit fields of levels t and t-dt in one
     word) is what holds the FP32 loop near the FP64 oracle: with it the worst
     seismogram misfit of u and of w is at least 3x smaller than with the
-    plain FP32 update (SEMSKS_F_NO_KAHAN) on the same 4000 steps."""
+    plain FP32 update (SEMSKS_F_NO_KAHAN) on the same 4000 steps.  The plain
+    update reads and writes no compensation words; it still stays within the
+    FP32 drift bound of 1e-4 that the reference's own arithmetic meets."""
     print("case: 200x100 elements, 4000 steps, compensated vs plain FP32 update", flush=True)
     comp = _worst_errors(tmp_path / "k", monkeypatch, 0)
     plain = _worst_errors(tmp_path / "p", monkeypatch, semsks.F_NO_KAHAN)
     print("worst rel-L2 u, w: compensated %.2e %.2e, plain %.2e %.2e" % (comp[0], comp[1], plain[0], plain[1]))
     for c, p in zip(comp, plain):
         assert c * 3.0 <= p, (comp, plain)
+        assert p <= 1e-4, plain
