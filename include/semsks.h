@@ -56,7 +56,8 @@ enum {
 
 /* problem flags */
 enum {
-    SEMSKS_F_NO_KAHAN = 1     /* plain FP32 displacement update (for error studies) */
+    SEMSKS_F_NO_KAHAN = 1     /* plain FP32 displacement update, the reference's own arithmetic
+                                 (par key kahan=0): no compensation words read or written */
 };
 
 /* Number of floats in one material record: rho followed by the 15 distinct
