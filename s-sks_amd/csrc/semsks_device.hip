@@ -2022,7 +2022,8 @@ static int nt_sem_of(const semsks_problem *p) { return (int)((float)((p->nt - 3)
 size_t semsks_footprint(const semsks_problem *p, int shared)
 {
     const size_t nodes = plane_of(p);
-    size_t b = nodes * 4 * (3 * 4);                 /* three levels: xyz plane + compensation words */
+    /* three levels: xyz plane + compensation words (none for the plain update) */
+    size_t b = nodes * 4 * 3 * ((p->flags & SEMSKS_F_NO_KAHAN) ? 3 : 4);
     if (!shared) b += nodes * 2 + (size_t)p->nex * p->nez * 2 +
                       (size_t)p->n_mat * (64 + 4 * (p->hpn + 1) * (p->hpn + 1)) +
                       (size_t)semsks_n_edge(nx_of(p), nz_of(p), p->hpn) * 24 +
@@ -2205,16 +2206,17 @@ semsks_status semsks_admit(semsks_engine *e, const semsks_problem *p, int share,
     if (st != SEMSKS_OK) { s = Slot(); return st; }
     {
         float *st_all = nullptr;
-        /* three levels, each an xyz plane and a compensation-word plane;
-           every plane starts on a 256-byte boundary (nodes is a multiple of
-           64) */
+        /* three levels, each an xyz plane and, with compensation, a
+           compensation-word plane (the plain update never touches the
+           words: no planes, comp[l] null); every plane starts on a 256-byte
+           boundary (nodes is a multiple of 64) */
         d.nlev = 3;
-        const int nplanes = d.nlev * 4;
+        const int nplanes = d.nlev * (d.kahan ? 4 : 3);
         ACHK(dalloc(s, &st_all, nodes * nplanes));
         ACHK(hipMemsetAsync(st_all, 0, nodes * nplanes * sizeof(float), e->setup));
         for (int l = 0; l < d.nlev; ++l) {
             d.u[l] = st_all + 3 * l * nodes;
-            d.comp[l] = reinterpret_cast<uint32_t *>(st_all + 3 * d.nlev * nodes + l * nodes);
+            d.comp[l] = d.kahan ? reinterpret_cast<uint32_t *>(st_all + 3 * d.nlev * nodes + l * nodes) : nullptr;
         }
     }
     if (p->atten > 0) {   /* atten_taper (sem2d_psvsh.c:1725-1738), double then float */

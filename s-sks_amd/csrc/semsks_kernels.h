@@ -11,9 +11,9 @@
  *             tile upward reads and writes one contiguous run per plane
  *   comp[l]   one 32-bit word per node: the Kahan compensations of the three
  *             components at level l and at level l-1, signed 5-bit fields in
- *             units of ulp(u)/32 (without Kahan no kernel reads or writes
-             them: the planes stay zero and a step moves 36 bytes of state
-             per node instead of 44)
+ *             units of ulp(u)/32 (without Kahan there are no word planes,
+             comp[l] is null and no kernel reads or writes words: a step
+             moves 36 bytes of state per node instead of 44)
  *   mat_id    16-bit material record id per node (same node order)
  *   emat      16-bit record id per element when all its nodes share one
  *             record, 0xFFFF otherwise
